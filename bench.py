@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Fused block-SpMV benchmark on one MI355X.
+
+Workload (the kernel-bench configuration): random banded sparse matrix with
+4,194,304 rows and 32 nonzeros per row (the diagonal 40 plus 31 distinct
+columns drawn uniformly within +-50,000 of the diagonal, values U(-1,1)),
+stored as SELL-C-sigma with C=32, sigma=256, FP64.  One step is one
+phist_DfusedSpmv_mvdot call: W = A*V (alpha=1, beta=0) with both column
+dots <W,W> and <V,W>, V an 8-column block of KISS random numbers
+(phist_Dmvec_random).  Inputs are resident in HBM when timing starts.
+
+Algorithmic bytes per step (compulsory traffic, x read once, dots free):
+    nnz*(8+4) + 4*N + N*nvec*8*2
+`value` divides them by the wall-clock time per step; roofline.achieved
+divides them by the SpMV kernel's own average duration, taken with HIP
+events on the library stream during the timed steps.
+
+cpu_baseline: PHIST's own builtin leaf kernels (dspmvm_8 + two ddot_8,
+compiled from the reference sources by oracle/Makefile, OpenMP) on the
+first quarter of the same matrix, timed on this host; if that build is
+absent, the oracle's single-threaded C restatement.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fused block-SpMV GB/s (% HBM peak); JD time-to-1e-8 mixed vs FP64; p99"
+
+
+def spmv_bytes(nnz, n, nvec, sval=8, sx=8, beta_nonzero=False):
+    return nnz * (sval + 4) + 4 * n + n * nvec * sx * (2 + (1 if beta_nonzero else 0))
+
+
+def read_traffic(path):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None.
+    The file holds lines `traffic_bytes_per_launch: <float>`."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(A, n_sample, nvec, budget_s=12.0):
+    """builtin dspmvm_8 + ddot_8 x2 on the first n_sample rows"""
+    import oracle_lib as ol
+    ro, ci, va = A.crs()
+    nnz_s = int(ro[n_sample] - 1)
+    ro_s = np.ascontiguousarray(ro[:n_sample + 1])
+    ci_s = np.ascontiguousarray(ci[:nnz_s])
+    va_s = np.ascontiguousarray(va[:nnz_s])
+    del ro, ci, va
+    ncols = int(ci_s.max())  # 1-based: x rows 0..ncols-1 are read
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (ncols, nvec))
+    y = np.zeros((n_sample, nvec))
+    ref = ol.ref_leaf()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    I, L, D = ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    p = ol.p
+    if ref is not None and nvec == 8:
+        kind = "reference"
+        halo_ptr = ro_s[1:].copy()
+        halo = np.zeros(8)
+        shifts = np.zeros(8)
+        ww = np.zeros(8)
+        vw = np.zeros(8)
+
+        def step():
+            ref.dspmvm_8_(ctypes.byref(I(n_sample)), ctypes.byref(I(0)), ctypes.byref(I(ncols)),
+                          ctypes.byref(L(nnz_s)), ctypes.byref(D(1.0)), p(ro_s), p(halo_ptr), p(ci_s), p(va_s),
+                          p(shifts), p(x), p(halo), ctypes.byref(D(0.0)), p(y))
+            ref.ddot_8_(ctypes.byref(I(n_sample)), p(x), p(y), p(vw))
+            ref.ddot_8_(ctypes.byref(I(n_sample)), p(y), p(y), p(ww))
+    else:
+        kind = "port"
+        threads = 1
+        crs = ol.Crs(n_sample, ncols, ro_s, ci_s, va_s)
+
+        def step():
+            ol.spmv(crs, 1.0, x, 0.0, y, nt=True)
+            ol.dot(x[:n_sample], y)
+            ol.dot(y, y)
+    step()
+    times = []
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s or len(times) < 3:
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 200:
+            break
+    t = float(np.median(times))
+    gbs = spmv_bytes(nnz_s, n_sample, nvec) / t / 1e9
+    sample = (f"first {n_sample} rows ({nnz_s} nnz) of the same matrix, nvec={nvec}, "
+              f"spmv (alpha=1, beta=0) + 2 column dots, median of {len(times)} runs")
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": threads, "kind": kind, "sample": sample,
+            "ms_per_step": round(t * 1e3, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nvec", type=int, default=8)
+    ap.add_argument("--rows", type=int, default=4194304)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import phist_amd
+    be = phist_amd.Backend(np.float64)
+    be.init()
+    L = be.L
+    iflag = ctypes.c_int(0)
+    L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
+    N, nvec = args.rows, args.nvec
+    t0 = time.perf_counter()
+    A = be.banded_random(N, 32, 50000, 40.0, 2024)
+    t_build = time.perf_counter() - t0
+    m = be.map(N)
+    x = be.mvec(m, nvec)
+    x.random()
+    y = be.mvec(m, nvec)
+    nbytes = spmv_bytes(A.nnz, N, nvec)
+    print(f"bench: matrix {N} rows, {A.nnz} nnz built in {t_build:.1f} s; nvec={nvec}", file=sys.stderr, flush=True)
+
+    ww = np.zeros(nvec)
+    vw = np.zeros(nvec)
+    pw, pv = ww.ctypes.data_as(ctypes.c_void_p), vw.ctypes.data_as(ctypes.c_void_p)
+    f = be.fn("fused_spmv_mvdot")
+    one, zero = ctypes.c_double(1.0), ctypes.c_double(0.0)
+
+    def step():
+        fl = ctypes.c_int(0)
+        f(one, A.h, x.h, zero, y.h, pw, pv, ctypes.byref(fl))
+        if fl.value:
+            raise RuntimeError(f"fused_spmv_mvdot iflag={fl.value}")
+
+    for _ in range(args.warmup):
+        step()
+    be.synchronize()
+    L.phist_amd_kernel_timer(1, ctypes.byref(iflag))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    be.synchronize()
+    t1 = time.perf_counter()
+    kms = ctypes.c_double(0.0)
+    kcount = ctypes.c_int64(0)
+    L.phist_amd_kernel_timer_read(ctypes.byref(kms), ctypes.byref(kcount), ctypes.byref(iflag))
+    L.phist_amd_kernel_timer(0, ctypes.byref(iflag))
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    k_avg_ms = kms.value / max(1, kcount.value)
+    value = nbytes / (ms_step * 1e-3) / 1e9
+    achieved = nbytes / (k_avg_ms * 1e-3) / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"sell_spmv_fused_nvec{nvec}_traffic.txt"))
+
+    cpu = None
+    if not args.no_cpu:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(A, N // 4, nvec)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (banded random matrix from a seeded generator, KISS random block vector)",
+        "config": {
+            "workload": "fused_spmv_mvdot, banded random 4M rows x 32 nnz/row (+-50k band, diag 40), "
+                        "SELL-C-sigma C=32 sigma=256, FP64, alpha=1 beta=0, both dots",
+            "rows": N, "nnz": A.nnz, "nvec": nvec, "bytes_per_step": nbytes,
+            "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 1),
+            "kernel_avg_ms": round(k_avg_ms, 4), "kernel_launches_timed": kcount.value,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    for o in (x, y, A):
+        o.delete()
+    m.delete()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+/*
+ * phist_amd_ext.h -- extensions of libphist_kernels_amd.so.
+ *
+ * None of these has a counterpart in the reference's kernel interface.  They
+ * are for tuning, measurement and inspection (benchmarks, parity tests) and
+ * for building test problems; solvers written against phist_amd.h never need
+ * them.
+ */
+#ifndef PHIST_AMD_EXT_H
+#define PHIST_AMD_EXT_H
+
+#include "phist_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* SELL-C-sigma parameters for matrices created after the call
+ * (C a power of two, 0 = default 32; sigma 0 = default 256). */
+void phist_amd_sparseMat_set_format(int C, int sigma, int* iflag);
+/* upper bound for the persistent grid of the streaming kernels (0 = one
+ * resident wave of workgroups on every CU) */
+void phist_amd_set_grid_blocks(int nblocks, int* iflag);
+
+/* the library's HIP stream (hipStream_t), device synchronisation, and an
+ * event timer on that stream (ms between start and stop) */
+void phist_amd_get_stream(void** stream, int* iflag);
+void phist_amd_synchronize(int* iflag);
+void phist_amd_timer_start(int* iflag);
+void phist_amd_timer_stop(float* ms, int* iflag);
+void phist_amd_device_info(char* name, int name_len, int* num_cu, int* iflag);
+/* per-kernel timer: with enable != 0 the main kernel of every SpMV and
+ * tall-skinny GEMM call is bracketed by HIP events on the library stream
+ * (synchronising after it); read returns the summed milliseconds and the
+ * number of timed launches since the last enable */
+void phist_amd_kernel_timer(int enable, int* iflag);
+void phist_amd_kernel_timer_read(double* total_ms, int64_t* count, int* iflag);
+
+/* the global KISS stream: reset to the initial state, jump ahead by n draws,
+ * draw n doubles in [-1,1) */
+void phist_amd_kiss_reset(int* iflag);
+void phist_amd_kiss_skip(uint64_t n, int* iflag);
+void phist_amd_kiss_draw(int64_t n, double* out, int* iflag);
+
+/* host copies of a matrix's index data.  get_crs: builtin-order CRS with
+ * 1-based row_offset (nrows+1) and col_idx (nnz).  get_sell: C, sigma,
+ * chunk count and padded slot count; arrays may be NULL to query sizes only
+ * (chunk_ptr nchunks+1, chunk_w nchunks, perm nchunks*C, col/val nslots). */
+void phist_Damd_sparseMat_get_crs(phist_Dconst_sparseMat_ptr A, int64_t* row_offset, int32_t* col_idx, double* val,
+                                  int* iflag);
+void phist_Samd_sparseMat_get_crs(phist_Sconst_sparseMat_ptr A, int64_t* row_offset, int32_t* col_idx, float* val,
+                                  int* iflag);
+void phist_Damd_sparseMat_get_sell(phist_Dconst_sparseMat_ptr A, int* C, int* sigma, int64_t* nchunks,
+                                   int64_t* nslots, int64_t* chunk_ptr, int32_t* chunk_w, int32_t* perm,
+                                   int32_t* col, double* val, int* iflag);
+void phist_Samd_sparseMat_get_sell(phist_Sconst_sparseMat_ptr A, int* C, int* sigma, int64_t* nchunks,
+                                   int64_t* nslots, int64_t* chunk_ptr, int32_t* chunk_w, int32_t* perm,
+                                   int32_t* col, float* val, int* iflag);
+
+/* test-problem row functions (phist_sparseMat_rowFunc); the _f variants
+ * write float values. */
+typedef struct {
+  int64_t n;          /* rows = columns */
+  int32_t nnz_per_row;/* including the diagonal */
+  int64_t halfband;   /* off-diagonal columns within +-halfband */
+  double diag;        /* diagonal value */
+  uint64_t seed;
+} phist_amd_banded_params;
+typedef struct {
+  int64_t nx, ny, nz;
+} phist_amd_laplace3d_params;
+int phist_amd_banded_random_rowfunc(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+int phist_amd_banded_random_rowfunc_f(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+int phist_amd_laplace3d_rowfunc(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+int phist_amd_laplace3d_rowfunc_f(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+int phist_amd_rowfunc_shared_ctor(void* arg, void** work);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHIST_AMD_EXT_H */

@@ -1,0 +1,328 @@
+"""ctypes binding of libphist_kernels_amd.so, the MI355X kernel library for PHIST.
+
+This plays the role of PHIST's own ctypes loader (python/phist_kernels.py in
+the PHIST sources, which loads libphist_kernels_<lib>.so): no arithmetic
+happens in Python, every call goes through the C ABI declared in
+include/phist_amd.h.  The thin object wrappers (Map, Mvec, SdMat, SparseMat)
+only hold handles and convert numpy arrays at the boundary.
+
+The library needs a HIP device; importing this module works without one (the
+shared object loads), but the first call that touches the GPU fails with the
+library's own error code.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "Backend", "PhistError", "LIB_PATH", "build"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphist_kernels_amd.so")
+
+# return codes (include/phist_amd.h)
+SUCCESS = 0
+NOT_IMPLEMENTED = -99
+INVALID_INPUT = -55
+BAD_CAST = -88
+# input flags
+ROBUST_REDUCTIONS = 1
+NO_GLOBAL_REDUCTION = 2
+MVEC_RUN_ON_DEVICE = 2048
+ORTHOG_RANDOMIZE_NULLSPACE = 131072
+
+ROWFUNC = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                           ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p)
+
+
+class PhistError(RuntimeError):
+    def __init__(self, name, code):
+        super().__init__(f"{name} returned iflag={code}")
+        self.code = code
+
+
+class BandedParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nnz_per_row", ctypes.c_int32), ("halfband", ctypes.c_int64),
+                ("diag", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class Laplace3dParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def build(jobs=8):
+    """Compile the library in-tree (hipcc for gfx950)."""
+    import subprocess
+    subprocess.check_call(["make", "-C", os.path.join(HERE, "csrc"), f"-j{jobs}"])
+
+
+def lib():
+    """The loaded shared library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C phist_amd/csrc` "
+                               "(or __graft_entry__.build())")
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Backend:
+    """Typed front end: dtype float64 -> phist_D*, float32 -> phist_S*."""
+
+    def __init__(self, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        if self.dtype == np.float64:
+            self.p, self.ct = "D", ctypes.c_double
+        elif self.dtype == np.float32:
+            self.p, self.ct = "S", ctypes.c_float
+        else:
+            raise ValueError("dtype must be float64 or float32")
+        self.L = lib()
+
+    # -- raw calls -----------------------------------------------------------
+    def fn(self, name):
+        return getattr(self.L, f"phist_{self.p}{name}")
+
+    def call(self, name, *args, flags=0, allow=(0,)):
+        iflag = ctypes.c_int(flags)
+        self.fn(name)(*args, ctypes.byref(iflag))
+        if iflag.value not in allow and iflag.value < 0:
+            raise PhistError(f"phist_{self.p}{name}", iflag.value)
+        return iflag.value
+
+    def gcall(self, name, *args, flags=0):
+        """type-independent call phist_<name>"""
+        iflag = ctypes.c_int(flags)
+        getattr(self.L, f"phist_{name}")(*args, ctypes.byref(iflag))
+        if iflag.value < 0:
+            raise PhistError(f"phist_{name}", iflag.value)
+        return iflag.value
+
+    def s(self, x):
+        return self.ct(x)
+
+    # -- objects -------------------------------------------------------------
+    def init(self):
+        self.gcall("kernels_init", None, None)
+
+    def map(self, n):
+        return Map(self, n)
+
+    def mvec(self, map_, nvec, data=None):
+        v = Mvec(self, map_, nvec)
+        if data is not None:
+            v.set(data)
+        return v
+
+    def sdmat(self, nrows, ncols, data=None):
+        m = SdMat(self, nrows, ncols)
+        if data is not None:
+            m.set(data)
+        return m
+
+    def read_mm(self, path):
+        h = ctypes.c_void_p()
+        self.call("sparseMat_read_mm", ctypes.byref(h), None, path.encode())
+        return SparseMat(self, h)
+
+    def from_rowfunc(self, n, maxnne, rowfunc, arg, ncols=None, parallel=False):
+        """rowfunc: a ctypes function pointer (ROWFUNC or a library symbol)."""
+        h = ctypes.c_void_p()
+        nc = n if ncols is None else ncols
+        if parallel:
+            ctor = self.L.phist_amd_rowfunc_shared_ctor
+            self.call("sparseMat_create_fromRowFuncWithConstructor", ctypes.byref(h), None, ctypes.c_int64(n),
+                      ctypes.c_int64(nc), ctypes.c_int32(maxnne), rowfunc, ctor, arg)
+        else:
+            self.call("sparseMat_create_fromRowFunc", ctypes.byref(h), None, ctypes.c_int64(n), ctypes.c_int64(nc),
+                      ctypes.c_int32(maxnne), rowfunc, arg)
+        return SparseMat(self, h)
+
+    def banded_random(self, n, nnz_per_row, halfband, diag, seed, parallel=True):
+        self._banded = BandedParams(n, nnz_per_row, halfband, diag, seed)
+        f = self.L.phist_amd_banded_random_rowfunc if self.p == "D" else self.L.phist_amd_banded_random_rowfunc_f
+        return self.from_rowfunc(n, nnz_per_row, f, ctypes.byref(self._banded), parallel=parallel)
+
+    def laplace3d(self, nx, ny, nz):
+        self._lap = Laplace3dParams(nx, ny, nz)
+        f = self.L.phist_amd_laplace3d_rowfunc if self.p == "D" else self.L.phist_amd_laplace3d_rowfunc_f
+        return self.from_rowfunc(nx * ny * nz, 7, f, ctypes.byref(self._lap), parallel=True)
+
+    # -- kernels (thin) ------------------------------------------------------
+    def spmv(self, alpha, A, x, beta, y):
+        self.call("sparseMat_times_mvec", self.s(alpha), A.h, x.h, self.s(beta), y.h)
+
+    def spmv_shifted(self, alpha, A, shifts, x, beta, y):
+        sh = np.ascontiguousarray(shifts, dtype=self.dtype)
+        self.call("sparseMat_times_mvec_vadd_mvec", self.s(alpha), A.h, _ptr(sh), x.h, self.s(beta), y.h)
+
+    def fused_spmv_mvdot(self, alpha, A, V, beta, W, want_ww=True, want_vw=True):
+        ww = np.zeros(V.nvec, self.dtype)
+        vw = np.zeros(V.nvec, self.dtype)
+        self.call("fused_spmv_mvdot", self.s(alpha), A.h, V.h, self.s(beta), W.h,
+                  _ptr(ww) if want_ww else None, _ptr(vw) if want_vw else None)
+        return (ww if want_ww else None), (vw if want_vw else None)
+
+    def dot(self, V, W):
+        out = np.zeros(V.nvec, self.dtype)
+        self.call("mvec_dot_mvec", V.h, W.h, _ptr(out))
+        return out
+
+    def norm2(self, V):
+        out = np.zeros(V.nvec, self.dtype)
+        self.call("mvec_norm2", V.h, _ptr(out))
+        return out
+
+    def gram(self, alpha, V, W, beta, C):
+        self.call("mvecT_times_mvec", self.s(alpha), V.h, W.h, self.s(beta), C.h)
+
+    def times_sdmat(self, alpha, V, C, beta, W):
+        self.call("mvec_times_sdMat", self.s(alpha), V.h, C.h, self.s(beta), W.h)
+
+    def times_sdmat_inplace(self, V, M):
+        self.call("mvec_times_sdMat_inplace", V.h, M.h)
+
+    def synchronize(self):
+        self.gcall("amd_synchronize")
+
+
+class Map:
+    def __init__(self, be, n):
+        self.be, self.n = be, n
+        self.h = ctypes.c_void_p()
+        be.gcall("map_create", ctypes.byref(self.h), None, ctypes.c_int64(n))
+
+    def delete(self):
+        if self.h:
+            self.be.gcall("map_delete", self.h)
+            self.h = ctypes.c_void_p()
+
+
+class Mvec:
+    def __init__(self, be, map_, nvec, handle=None):
+        self.be, self.map, self.nvec = be, map_, nvec
+        self.n = map_.n if map_ is not None else None
+        if handle is None:
+            self.h = ctypes.c_void_p()
+            be.call("mvec_create", ctypes.byref(self.h), map_.h, ctypes.c_int(nvec))
+        else:
+            self.h = handle
+
+    def set(self, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.be.dtype).reshape(self.n, self.nvec))
+        self.be.call("mvec_set_data", self.h, _ptr(a), ctypes.c_int32(self.nvec), ctypes.c_int(1))
+
+    def get(self):
+        a = np.empty((self.n, self.nvec), dtype=self.be.dtype)
+        self.be.call("mvec_get_data", self.h, _ptr(a), ctypes.c_int32(self.nvec), ctypes.c_int(1))
+        return a
+
+    def view(self, jmin, jmax):
+        vh = ctypes.c_void_p()
+        self.be.call("mvec_view_block", self.h, ctypes.byref(vh), ctypes.c_int(jmin), ctypes.c_int(jmax))
+        return Mvec(self.be, self.map, jmax - jmin + 1, handle=vh)
+
+    def random(self):
+        self.be.call("mvec_random", self.h)
+
+    def put_value(self, v):
+        self.be.call("mvec_put_value", self.h, self.be.s(v))
+
+    def delete(self):
+        if self.h:
+            self.be.call("mvec_delete", self.h)
+            self.h = ctypes.c_void_p()
+
+
+class SdMat:
+    def __init__(self, be, nrows, ncols, handle=None):
+        self.be, self.nrows, self.ncols = be, nrows, ncols
+        if handle is None:
+            self.h = ctypes.c_void_p()
+            be.call("sdMat_create", ctypes.byref(self.h), ctypes.c_int(nrows), ctypes.c_int(ncols), None)
+        else:
+            self.h = handle
+
+    def set(self, a):
+        a = np.asfortranarray(np.asarray(a, dtype=self.be.dtype).reshape(self.nrows, self.ncols))
+        self.be.call("sdMat_set_data", self.h, _ptr(a), ctypes.c_int32(max(1, self.nrows)), ctypes.c_int(0))
+
+    def get(self):
+        a = np.zeros((self.nrows, self.ncols), dtype=self.be.dtype, order="F")
+        self.be.call("sdMat_from_device", self.h)
+        self.be.call("sdMat_get_data", self.h, _ptr(a), ctypes.c_int32(max(1, self.nrows)), ctypes.c_int(0))
+        return np.ascontiguousarray(a)
+
+    def delete(self):
+        if self.h:
+            self.be.call("sdMat_delete", self.h)
+            self.h = ctypes.c_void_p()
+
+
+class SparseMat:
+    def __init__(self, be, h):
+        self.be, self.h = be, h
+        n = ctypes.c_int64()
+        be.call("sparseMat_global_nrows", h, ctypes.byref(n))
+        self.nrows = n.value
+        be.call("sparseMat_global_ncols", h, ctypes.byref(n))
+        self.ncols = n.value
+        be.call("sparseMat_global_nnz", h, ctypes.byref(n))
+        self.nnz = n.value
+        m = ctypes.c_void_p()
+        be.call("sparseMat_get_domain_map", h, ctypes.byref(m))
+        self.domain_map = _BorrowedMap(self.ncols, m)
+        be.call("sparseMat_get_range_map", h, ctypes.byref(m))
+        self.range_map = _BorrowedMap(self.nrows, m)
+
+    def crs(self):
+        """builtin-order CRS (1-based row_offset, col_idx; values)"""
+        ro = np.empty(self.nrows + 1, np.int64)
+        ci = np.empty(max(1, self.nnz), np.int32)
+        va = np.empty(max(1, self.nnz), self.be.dtype)
+        iflag = ctypes.c_int(0)
+        getattr(self.be.L, f"phist_{self.be.p}amd_sparseMat_get_crs")(self.h, _ptr(ro), _ptr(ci), _ptr(va),
+                                                                       ctypes.byref(iflag))
+        if iflag.value:
+            raise PhistError("get_crs", iflag.value)
+        return ro, ci[:self.nnz], va[:self.nnz]
+
+    def sell(self):
+        C, sig = ctypes.c_int(), ctypes.c_int()
+        nch, nsl = ctypes.c_int64(), ctypes.c_int64()
+        f = getattr(self.be.L, f"phist_{self.be.p}amd_sparseMat_get_sell")
+        iflag = ctypes.c_int(0)
+        f(self.h, ctypes.byref(C), ctypes.byref(sig), ctypes.byref(nch), ctypes.byref(nsl), None, None, None, None,
+          None, ctypes.byref(iflag))
+        if iflag.value:
+            raise PhistError("get_sell", iflag.value)
+        cp = np.empty(nch.value + 1, np.int64)
+        cw = np.empty(max(1, nch.value), np.int32)
+        pm = np.empty(max(1, nch.value * C.value), np.int32)
+        co = np.empty(max(1, nsl.value), np.int32)
+        va = np.empty(max(1, nsl.value), self.be.dtype)
+        f(self.h, None, None, None, None, _ptr(cp), _ptr(cw), _ptr(pm), _ptr(co), _ptr(va), ctypes.byref(iflag))
+        if iflag.value:
+            raise PhistError("get_sell", iflag.value)
+        return dict(C=C.value, sigma=sig.value, nchunks=nch.value, nslots=nsl.value, chunk_ptr=cp,
+                    chunk_w=cw[:nch.value], perm=pm[:nch.value * C.value], col=co[:nsl.value], val=va[:nsl.value])
+
+    def delete(self):
+        if self.h:
+            self.be.call("sparseMat_delete", self.h)
+            self.h = ctypes.c_void_p()
+
+
+class _BorrowedMap:
+    """a map owned by a matrix (not deleted by the caller)"""
+
+    def __init__(self, n, h):
+        self.n, self.h = n, h

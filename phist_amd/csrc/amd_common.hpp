@@ -1,0 +1,260 @@
+// amd_common.hpp -- internal object model of libphist_kernels_amd.
+//
+// Objects behind the opaque handles of include/phist_amd.h:
+//   Comm       single-process communicator (rank 0 of 1)
+//   Map        contiguous row distribution (nglob rows, all local)
+//   Context    row/col/range/domain maps of a sparse matrix
+//   Mvec<T>    device-resident row-major block vector with a lazily created
+//              pinned host mirror; views share the root's storage
+//   SdMat<T>   column-major small dense matrix mirrored on host and device
+//   SparseMat<T>  SELL-C-sigma on the device + the builtin-order CRS on the host
+//
+// Every device kernel runs on the library's stream (rt().stream); every entry
+// point returns only after the results it hands to the host are complete.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include <string>
+#include <stdexcept>
+
+#include "phist_amd_ext.h"
+
+namespace phx {
+
+// ----------------------------------------------------------------------------
+// error handling: every entry point converts exceptions to iflag codes
+// ----------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...);
+
+#define PHX_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      ::phx::fail(PHIST_FUNCTIONAL_ERROR, "%s:%d: HIP error %s in %s", __FILE__, __LINE__, \
+                  hipGetErrorString(e_), #call);                                       \
+  } while (0)
+
+#define PHX_REQUIRE(cond, code, ...)          \
+  do {                                        \
+    if (!(cond)) ::phx::fail(code, __VA_ARGS__); \
+  } while (0)
+
+// Runs body, maps exceptions to *iflag and prints the message unless quiet.
+template <class F>
+inline void guard(int* iflag, const char* fn, F&& body) {
+  try {
+    body();
+  } catch (const Error& e) {
+    if (iflag) *iflag = e.code;
+    if (e.code < 0 && e.code != PHIST_NOT_IMPLEMENTED) fprintf(stderr, "[phist_amd] %s: %s\n", fn, e.what());
+  } catch (const std::bad_alloc&) {
+    if (iflag) *iflag = PHIST_MEM_ALLOC_FAILED;
+    fprintf(stderr, "[phist_amd] %s: host allocation failed\n", fn);
+  } catch (const std::exception& e) {
+    if (iflag) *iflag = PHIST_CAUGHT_EXCEPTION;
+    fprintf(stderr, "[phist_amd] %s: %s\n", fn, e.what());
+  }
+}
+
+// ----------------------------------------------------------------------------
+// runtime: device, stream, workspace
+// ----------------------------------------------------------------------------
+struct Runtime {
+  bool initialized = false;
+  int device = 0;
+  int num_cu = 256;
+  int num_xcd = 8;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  // device scratch for block partials of reductions (grown on demand)
+  void* d_work = nullptr;
+  size_t d_work_bytes = 0;
+  // pinned host scratch for reduction results
+  void* h_work = nullptr;
+  size_t h_work_bytes = 0;
+  // default SELL parameters for newly built matrices (0 = automatic)
+  int sell_C = 0, sell_sigma = 0;
+  // maximum persistent grid size for streaming kernels (0 = automatic)
+  int grid_blocks = 0;
+  // optional timing of the main kernel of each hot-path call (HIP events on
+  // the library stream around that launch only)
+  bool ktimer_on = false;
+  hipEvent_t kt_start = nullptr, kt_stop = nullptr;
+  double ktimer_ms = 0.0;
+  int64_t ktimer_count = 0;
+};
+Runtime& rt();
+void ensure_init();
+void* device_work(size_t bytes);
+void* host_work(size_t bytes);
+void sync_stream();
+// brackets one kernel launch when the kernel timer is on
+void ktimer_begin();
+void ktimer_end();  // synchronises the event and accumulates the time
+
+// ----------------------------------------------------------------------------
+// communicator, map, context
+// ----------------------------------------------------------------------------
+struct Comm {
+  int rank = 0, size = 1;
+};
+Comm* default_comm();
+
+struct Map {
+  const Comm* comm = nullptr;
+  int64_t nglob = 0;
+  int32_t nloc = 0;
+  int64_t ilower = 0;
+};
+
+struct Context {
+  const Map* row_map = nullptr;
+  const Map* col_map = nullptr;
+  const Map* range_map = nullptr;
+  const Map* domain_map = nullptr;
+};
+
+// ----------------------------------------------------------------------------
+// block vectors
+// ----------------------------------------------------------------------------
+template <typename T>
+struct Mvec {
+  const Map* map = nullptr;
+  int64_t nrows = 0;
+  int nvec = 0;       // columns of this (view of the) block
+  int64_t lda = 0;    // row stride of the root allocation (elements)
+  Mvec* root = this;  // owner of the storage
+  int jofs = 0;       // first column inside the root
+  T* dev_base = nullptr;   // root only
+  T* host_base = nullptr;  // root only, pinned, created on demand
+  int nviews = 0;          // live views of this root
+  bool is_view() const { return root != this; }
+  T* dev() const { return root->dev_base + jofs; }
+  T* host();  // creates the root mirror on first use
+};
+
+// ----------------------------------------------------------------------------
+// small dense matrices (column-major, host + device copies)
+// ----------------------------------------------------------------------------
+template <typename T>
+struct SdMat {
+  int nrows = 0, ncols = 0;
+  int ld = 0;             // column stride of the root (elements)
+  SdMat* root = this;
+  int iofs = 0, jofs = 0;
+  T* host_base = nullptr;  // root only
+  T* dev_base = nullptr;   // root only
+  int nviews = 0;
+  bool is_view() const { return root != this; }
+  T* host() const { return root->host_base + (int64_t)jofs * root->ld + iofs; }
+  T* dev() const { return root->dev_base + (int64_t)jofs * root->ld + iofs; }
+  T& h(int i, int j) const { return host()[(int64_t)j * root->ld + i]; }
+};
+template <typename T> void sd_to_device(const SdMat<T>* M);
+template <typename T> void sd_from_device(const SdMat<T>* M);
+
+// ----------------------------------------------------------------------------
+// sparse matrices
+// ----------------------------------------------------------------------------
+template <typename T>
+struct SparseMat {
+  Map* row_map = nullptr;  // owned
+  Map* col_map = nullptr;  // owned (same layout as row_map for square matrices)
+  Map* range_map = nullptr;
+  Map* domain_map = nullptr;
+  Context ctx;
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  int32_t max_row_len = 0;
+  // builtin-order CRS on the host (0-based row_ptr and col, each row stably
+  // sorted by column); kept for index-parity checks and host-side consumers
+  std::vector<int64_t> crs_ptr;
+  std::vector<int32_t> crs_col;
+  std::vector<T> crs_val;
+  // SELL-C-sigma on the device
+  int C = 0, sigma = 0;
+  int64_t nchunks = 0;
+  int64_t nnz_padded = 0;
+  int64_t* d_chunk_ptr = nullptr;  // [nchunks+1] element offset of each chunk
+  int32_t* d_chunk_w = nullptr;    // [nchunks] chunk width (longest row)
+  int32_t* d_perm = nullptr;       // [nchunks*C] SELL row -> matrix row (-1: padding)
+  int32_t* d_col = nullptr;        // [nnz_padded] column of each slot
+  T* d_val = nullptr;              // [nnz_padded] value of each slot
+  bool perm_identity = true;
+  // host copies of the SELL index arrays (for inspection)
+  std::vector<int64_t> h_chunk_ptr;
+  std::vector<int32_t> h_chunk_w;
+  std::vector<int32_t> h_perm;
+};
+
+// build the SELL-C-sigma device copy of a matrix whose host CRS is filled
+template <typename T> void sell_build(SparseMat<T>* A, int C, int sigma);
+template <typename T> void sparse_free(SparseMat<T>* A);
+
+// handle casts
+template <typename T> struct Handles;
+template <> struct Handles<double> {
+  using mvec = phist_Dmvec; using sdMat = phist_DsdMat; using sparseMat = phist_DsparseMat;
+};
+template <> struct Handles<float> {
+  using mvec = phist_Smvec; using sdMat = phist_SsdMat; using sparseMat = phist_SsparseMat;
+};
+template <typename T> inline Mvec<T>* MV(typename Handles<T>::mvec* p) { return reinterpret_cast<Mvec<T>*>(p); }
+template <typename T> inline const Mvec<T>* MV(const typename Handles<T>::mvec* p) { return reinterpret_cast<const Mvec<T>*>(p); }
+template <typename T> inline SdMat<T>* SD(typename Handles<T>::sdMat* p) { return reinterpret_cast<SdMat<T>*>(p); }
+template <typename T> inline const SdMat<T>* SD(const typename Handles<T>::sdMat* p) { return reinterpret_cast<const SdMat<T>*>(p); }
+template <typename T> inline SparseMat<T>* SP(typename Handles<T>::sparseMat* p) { return reinterpret_cast<SparseMat<T>*>(p); }
+template <typename T> inline const SparseMat<T>* SP(const typename Handles<T>::sparseMat* p) { return reinterpret_cast<const SparseMat<T>*>(p); }
+
+// ----------------------------------------------------------------------------
+// device kernel launchers (spmv.hip, blas1.hip, tsgemm.hip)
+// ----------------------------------------------------------------------------
+struct SpmvRequest {
+  bool has_shift = false, has_beta = false;
+  bool dot_ww = false, dot_vw = false;
+  bool axpy_u = false;  // U = gamma*W + delta*U after W is formed
+};
+
+// y = alpha*(A + diag-shifts)*x + beta*y over column range of the given views;
+// optional column dots of the new y (ww) and of x with the new y (vw),
+// written to host arrays; optional U update (fused_spmv_mvdot_mvadd).
+template <typename T>
+void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mvec<T>* x, T beta, Mvec<T>* y,
+                 const SpmvRequest& req, T* dot_ww_host, T* dot_vw_host, T gamma = T(0), T delta = T(0),
+                 Mvec<T>* U = nullptr);
+
+// BLAS-1 on blocks
+template <typename T> void blas_put_value(Mvec<T>* V, T value);
+template <typename T> void blas_scale_cols(Mvec<T>* V, const T* scal_host, bool uniform);
+template <typename T> void blas_axpby(const T* alpha_host, bool alpha_uniform, const Mvec<T>* X, T beta, Mvec<T>* Y);
+template <typename T> void blas_elemwise(T alpha, const Mvec<T>* V, Mvec<T>* W);
+template <typename T> void blas_dot(const Mvec<T>* V, const Mvec<T>* W, T* out_host);
+template <typename T> void blas_copy(const Mvec<T>* X, Mvec<T>* Y);
+
+// tall-skinny GEMMs; the sdMat results are written to the device copy and
+// copied to the host copy before returning
+template <typename T> void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C);
+template <typename T> void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W);
+template <typename T> void gemm_times_sd_inplace(Mvec<T>* V, const SdMat<T>* M);
+template <typename T>
+void gemm_times_sd_add_times_sd(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W, const SdMat<T>* D);
+template <typename T>
+void gemm_fused_mvsdi_mvTmv(T alpha, const Mvec<T>* V, Mvec<T>* W, const SdMat<T>* C, T beta, SdMat<T>* D);
+template <typename T>
+void gemm_fused_mvsd_mvTmv(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W, SdMat<T>* WtW);
+
+// KISS random numbers (kiss.cpp)
+void kiss_reset();
+// draws n values of the global stream into out (double), advancing it by n
+void kiss_draw(int64_t n, double* out);
+void kiss_skip(uint64_t n);
+
+}  // namespace phx

@@ -1,0 +1,116 @@
+// amd_device.hpp -- device-side helpers shared by the HIP kernels.
+//
+// * persistent XCD-aware grids: MI355X dispatches workgroup b to XCD b % 8,
+//   so the tile space is split into 8 contiguous ranges, one per XCD, and the
+//   workgroups of an XCD walk their range together; the set of rows in flight
+//   on one XCD (and the x rows they gather) then stays narrow and L2-resident.
+// * 16-byte vector loads/stores of VPL consecutive scalars.
+// * deterministic two-stage reductions: each workgroup writes its partial
+//   sums, a single workgroup adds them in a fixed order.
+#pragma once
+#include "amd_common.hpp"
+
+namespace phx {
+
+constexpr int kXcds = 8;
+constexpr int kBlock = 256;  // threads per workgroup (4 wave64)
+
+// first and one-past-last tile of workgroup-walk for this block
+struct TileWalk {
+  int64_t t, end, stride;
+};
+__device__ inline TileWalk xcd_tiles(int64_t ntiles) {
+  const int xcd = blockIdx.x % kXcds;
+  const int k = blockIdx.x / kXcds;
+  const int per = gridDim.x / kXcds;
+  TileWalk w;
+  w.t = (int64_t)xcd * ntiles / kXcds + k;
+  w.end = (int64_t)(xcd + 1) * ntiles / kXcds;
+  w.stride = per;
+  return w;
+}
+
+template <typename T, int N>
+struct alignas(sizeof(T) * N) VecN {
+  T v[N];
+};
+
+template <typename T, int N>
+__device__ inline void vload(const T* p, T (&out)[N]) {
+  if constexpr (N == 1) {
+    out[0] = *p;
+  } else {
+    VecN<T, N> t = *reinterpret_cast<const VecN<T, N>*>(p);
+#pragma unroll
+    for (int e = 0; e < N; e++) out[e] = t.v[e];
+  }
+}
+
+template <typename T, int N>
+__device__ inline void vstore(T* p, const T (&in)[N]) {
+  if constexpr (N == 1) {
+    *p = in[0];
+  } else {
+    VecN<T, N> t;
+#pragma unroll
+    for (int e = 0; e < N; e++) t.v[e] = in[e];
+    *reinterpret_cast<VecN<T, N>*>(p) = t;
+  }
+}
+
+// sum over the lanes of a wave64 that share (lane % STRIDE)
+template <int STRIDE, typename T>
+__device__ inline T wave_sum_strided(T v) {
+#pragma unroll
+  for (int off = STRIDE; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Persistent-grid size for a kernel: resident workgroups, multiple of 8.
+template <typename K>
+inline int persistent_blocks(K kernel, size_t lds_bytes, int64_t ntiles) {
+  Runtime& r = rt();
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds_bytes) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  int64_t g = (int64_t)r.num_cu * per_cu;
+  if (r.grid_blocks > 0) g = std::min<int64_t>(g, r.grid_blocks);
+  int64_t need = ((ntiles + kXcds - 1) / kXcds) * kXcds;
+  g = std::min(g, std::max<int64_t>(need, kXcds));
+  g = std::max<int64_t>(kXcds, (g / kXcds) * kXcds);
+  return (int)g;
+}
+
+// Adds partials[b*width + i] over b in a fixed order; out[i] (device).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_reduce_partials(const T* __restrict__ partials, int nblocks, int width,
+                                                            T* __restrict__ out) {
+  __shared__ T sh[kBlock];
+  for (int i = 0; i < width; i++) {
+    T s = T(0);
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partials[(int64_t)b * width + i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+      if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = sh[0];
+    __syncthreads();
+  }
+}
+
+// reduce [nblocks][width] device partials and copy the width sums to host_out
+template <typename T>
+inline void reduce_partials_to_host(const T* d_partials, int nblocks, int width, T* host_out) {
+  Runtime& r = rt();
+  T* d_out = const_cast<T*>(d_partials) + (size_t)nblocks * width;
+  hipLaunchKernelGGL(k_reduce_partials<T>, dim3(1), dim3(kBlock), 0, r.stream, d_partials, nblocks, width, d_out);
+  PHX_HIP(hipGetLastError());
+  T* h = (T*)host_work(sizeof(T) * width);
+  PHX_HIP(hipMemcpyAsync(h, d_out, sizeof(T) * width, hipMemcpyDeviceToHost, r.stream));
+  PHX_HIP(hipStreamSynchronize(r.stream));
+  memcpy(host_out, h, sizeof(T) * width);
+}
+
+}  // namespace phx

@@ -1,0 +1,407 @@
+// runtime.cpp -- library initialisation, the single-process communicator,
+// contiguous maps, contexts, scratch memory and the HBM STREAM benchmarks.
+//
+// Reference behaviour: src/kernels/phist_kernels.h:79-185 (interface),
+// src/kernels/builtin/kernels.cpp:192-230 (init: thread setup + RNG init via
+// kernels_common_impl.cpp:23-88), src/kernels/builtin/map_module.F90:41-66,
+// 289-320 (contiguous distribution; ilower/iupper are 0-based).
+#include "amd_common.hpp"
+#include <cstdarg>
+#include <cstdlib>
+#include <algorithm>
+
+namespace phx {
+
+void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+Runtime& rt() {
+  static Runtime r;
+  return r;
+}
+
+void ensure_init() {
+  Runtime& r = rt();
+  if (r.initialized) return;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  PHX_REQUIRE(e == hipSuccess && ndev > 0, PHIST_FUNCTIONAL_ERROR,
+              "no HIP device available (the kernel library runs on the GPU only)");
+  PHX_HIP(hipGetDevice(&r.device));
+  hipDeviceProp_t prop;
+  PHX_HIP(hipGetDeviceProperties(&prop, r.device));
+  r.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  PHX_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+  PHX_HIP(hipEventCreate(&r.ev_start));
+  PHX_HIP(hipEventCreate(&r.ev_stop));
+  PHX_HIP(hipEventCreate(&r.kt_start));
+  PHX_HIP(hipEventCreate(&r.kt_stop));
+  if (const char* s = getenv("PHIST_AMD_SELL_C")) r.sell_C = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_SELL_SIGMA")) r.sell_sigma = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_GRID_BLOCKS")) r.grid_blocks = atoi(s);
+  kiss_reset();
+  r.initialized = true;
+}
+
+void* device_work(size_t bytes) {
+  Runtime& r = rt();
+  if (bytes > r.d_work_bytes) {
+    if (r.d_work) {
+      PHX_HIP(hipStreamSynchronize(r.stream));
+      PHX_HIP(hipFree(r.d_work));
+      r.d_work = nullptr;
+    }
+    size_t nb = std::max<size_t>(bytes, 1 << 20);
+    PHX_HIP(hipMalloc(&r.d_work, nb));
+    r.d_work_bytes = nb;
+  }
+  return r.d_work;
+}
+
+void* host_work(size_t bytes) {
+  Runtime& r = rt();
+  if (bytes > r.h_work_bytes) {
+    if (r.h_work) {
+      PHX_HIP(hipStreamSynchronize(r.stream));
+      PHX_HIP(hipHostFree(r.h_work));
+      r.h_work = nullptr;
+    }
+    size_t nb = std::max<size_t>(bytes, 1 << 16);
+    PHX_HIP(hipHostMalloc(&r.h_work, nb, hipHostMallocDefault));
+    r.h_work_bytes = nb;
+  }
+  return r.h_work;
+}
+
+void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
+
+void ktimer_begin() {
+  Runtime& r = rt();
+  if (r.ktimer_on) PHX_HIP(hipEventRecord(r.kt_start, r.stream));
+}
+
+void ktimer_end() {
+  Runtime& r = rt();
+  if (!r.ktimer_on) return;
+  PHX_HIP(hipEventRecord(r.kt_stop, r.stream));
+  PHX_HIP(hipEventSynchronize(r.kt_stop));
+  float ms = 0;
+  PHX_HIP(hipEventElapsedTime(&ms, r.kt_start, r.kt_stop));
+  r.ktimer_ms += ms;
+  r.ktimer_count++;
+}
+
+Comm* default_comm() {
+  static Comm c;
+  return &c;
+}
+
+// ---- STREAM kernels on HBM ----
+namespace {
+__global__ void k_stream_load(const double* __restrict__ a, int64_t n, double* out) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i];
+  if (s == 12345.678) out[0] = s;  // keeps the loads alive
+}
+__global__ void k_stream_store(double* __restrict__ a, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+__global__ void k_stream_copy(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+__global__ void k_stream_triad(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c,
+                               int64_t n, double s) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c[i] = a[i] + s * b[i];
+}
+
+// kind: 0 load, 1 store, 2 copy, 3 triad; returns mean/max GB/s over reps
+void run_stream(int kind, double* mean_bw, double* max_bw) {
+  ensure_init();
+  Runtime& r = rt();
+  const int64_t n = int64_t(1) << 27;  // 1 GiB per array
+  double *a = nullptr, *b = nullptr, *c = nullptr;
+  PHX_HIP(hipMalloc(&a, n * 8));
+  PHX_HIP(hipMalloc(&b, n * 8));
+  PHX_HIP(hipMalloc(&c, n * 8));
+  PHX_HIP(hipMemsetAsync(a, 0, n * 8, r.stream));
+  PHX_HIP(hipMemsetAsync(b, 0, n * 8, r.stream));
+  const int blocks = r.num_cu * 8, threads = 256;
+  const double bytes[4] = {8.0 * n, 8.0 * n, 16.0 * n, 24.0 * n};
+  const int reps = 20;
+  double sum = 0, best = 0;
+  for (int it = -2; it < reps; it++) {
+    PHX_HIP(hipEventRecord(r.ev_start, r.stream));
+    switch (kind) {
+      case 0: hipLaunchKernelGGL(k_stream_load, dim3(blocks), dim3(threads), 0, r.stream, a, n, c); break;
+      case 1: hipLaunchKernelGGL(k_stream_store, dim3(blocks), dim3(threads), 0, r.stream, a, n, 1.0); break;
+      case 2: hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(threads), 0, r.stream, a, b, n); break;
+      default: hipLaunchKernelGGL(k_stream_triad, dim3(blocks), dim3(threads), 0, r.stream, a, b, c, n, 3.0); break;
+    }
+    PHX_HIP(hipGetLastError());
+    PHX_HIP(hipEventRecord(r.ev_stop, r.stream));
+    PHX_HIP(hipEventSynchronize(r.ev_stop));
+    float ms = 0;
+    PHX_HIP(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
+    if (it < 0) continue;
+    double gbs = bytes[kind] / (ms * 1e-3) / 1e9;
+    sum += gbs;
+    best = std::max(best, gbs);
+  }
+  PHX_HIP(hipFree(a));
+  PHX_HIP(hipFree(b));
+  PHX_HIP(hipFree(c));
+  *mean_bw = sum / reps;
+  *max_bw = best;
+}
+}  // namespace
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" {
+
+void phist_kernels_init(int* argc, char*** argv, int* iflag) {
+  (void)argc;
+  (void)argv;
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    *iflag = 0;
+  });
+}
+
+void phist_kernels_finalize(int* iflag) {
+  guard(iflag, __func__, [&] {
+    Runtime& r = rt();
+    if (r.initialized) {
+      PHX_HIP(hipStreamSynchronize(r.stream));
+      if (r.d_work) PHX_HIP(hipFree(r.d_work));
+      if (r.h_work) PHX_HIP(hipHostFree(r.h_work));
+      PHX_HIP(hipEventDestroy(r.ev_start));
+      PHX_HIP(hipEventDestroy(r.ev_stop));
+      PHX_HIP(hipEventDestroy(r.kt_start));
+      PHX_HIP(hipEventDestroy(r.kt_stop));
+      PHX_HIP(hipStreamDestroy(r.stream));
+      r = Runtime();
+    }
+    *iflag = 0;
+  });
+}
+
+void phist_comm_create(phist_comm_ptr* comm, int* iflag) {
+  guard(iflag, __func__, [&] {
+    *comm = new Comm();
+    *iflag = 0;
+  });
+}
+void phist_comm_delete(phist_comm_ptr comm, int* iflag) {
+  guard(iflag, __func__, [&] {
+    if (comm != default_comm()) delete (Comm*)comm;
+    *iflag = 0;
+  });
+}
+void phist_comm_get_rank(phist_const_comm_ptr comm, int* rank, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(comm, PHIST_INVALID_INPUT, "NULL comm");
+    *rank = ((const Comm*)comm)->rank;
+    *iflag = 0;
+  });
+}
+void phist_comm_get_size(phist_const_comm_ptr comm, int* size, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(comm, PHIST_INVALID_INPUT, "NULL comm");
+    *size = ((const Comm*)comm)->size;
+    *iflag = 0;
+  });
+}
+void phist_comm_get_mpi_comm(phist_const_comm_ptr comm, MPI_Comm* mpiComm, int* iflag) {
+  guard(iflag, __func__, [&] {
+    (void)comm;
+    *mpiComm = 0;
+    *iflag = 0;
+  });
+}
+
+void phist_map_create(phist_map_ptr* map, phist_const_comm_ptr comm, phist_gidx nglob, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(nglob >= 0 && nglob <= INT32_MAX, PHIST_INVALID_INPUT, "map size %lld out of range", (long long)nglob);
+    Map* m = new Map();
+    m->comm = comm ? (const Comm*)comm : default_comm();
+    m->nglob = nglob;
+    m->nloc = (int32_t)nglob;
+    m->ilower = 0;
+    *map = m;
+    *iflag = 0;
+  });
+}
+void phist_map_delete(phist_map_ptr map, int* iflag) {
+  guard(iflag, __func__, [&] {
+    delete (Map*)map;
+    *iflag = 0;
+  });
+}
+void phist_map_get_comm(phist_const_map_ptr map, phist_const_comm_ptr* comm, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(map, PHIST_INVALID_INPUT, "NULL map");
+    *comm = ((const Map*)map)->comm;
+    *iflag = 0;
+  });
+}
+void phist_map_get_local_length(phist_const_map_ptr map, phist_lidx* nloc, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(map, PHIST_INVALID_INPUT, "NULL map");
+    *nloc = ((const Map*)map)->nloc;
+    *iflag = 0;
+  });
+}
+void phist_map_get_global_length(phist_const_map_ptr map, phist_gidx* nglob, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(map, PHIST_INVALID_INPUT, "NULL map");
+    *nglob = ((const Map*)map)->nglob;
+    *iflag = 0;
+  });
+}
+void phist_map_get_ilower(phist_const_map_ptr map, phist_gidx* ilower, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(map, PHIST_INVALID_INPUT, "NULL map");
+    *ilower = ((const Map*)map)->ilower;
+    *iflag = 0;
+  });
+}
+void phist_map_get_iupper(phist_const_map_ptr map, phist_gidx* iupper, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(map, PHIST_INVALID_INPUT, "NULL map");
+    const Map* m = (const Map*)map;
+    *iupper = m->ilower + m->nloc - 1;
+    *iflag = 0;
+  });
+}
+void phist_maps_compatible(phist_const_map_ptr map1, phist_const_map_ptr map2, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(map1 && map2, PHIST_INVALID_INPUT, "NULL map");
+    const Map* a = (const Map*)map1;
+    const Map* b = (const Map*)map2;
+    *iflag = (a == b || (a->nglob == b->nglob && a->nloc == b->nloc && a->ilower == b->ilower)) ? 0 : -1;
+  });
+}
+
+void phist_context_create(phist_context_ptr* ctx, phist_const_map_ptr row_map, phist_const_map_ptr col_map,
+                          phist_const_map_ptr range_map, phist_const_map_ptr domain_map, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(row_map, PHIST_INVALID_INPUT, "context needs a row map");
+    Context* c = new Context();
+    c->row_map = (const Map*)row_map;
+    c->col_map = (const Map*)col_map;
+    c->range_map = range_map ? (const Map*)range_map : c->row_map;
+    c->domain_map = domain_map ? (const Map*)domain_map : c->row_map;
+    *ctx = c;
+    *iflag = 0;
+  });
+}
+void phist_context_delete(phist_context_ptr vctx, int* iflag) {
+  guard(iflag, __func__, [&] {
+    delete (Context*)vctx;
+    *iflag = 0;
+  });
+}
+
+void phist_bench_stream_load(double* mean_bw, double* max_bw, int* iflag) {
+  guard(iflag, __func__, [&] { run_stream(0, mean_bw, max_bw); *iflag = 0; });
+}
+void phist_bench_stream_store(double* mean_bw, double* max_bw, int* iflag) {
+  guard(iflag, __func__, [&] { run_stream(1, mean_bw, max_bw); *iflag = 0; });
+}
+void phist_bench_stream_copy(double* mean_bw, double* max_bw, int* iflag) {
+  guard(iflag, __func__, [&] { run_stream(2, mean_bw, max_bw); *iflag = 0; });
+}
+void phist_bench_stream_triad(double* mean_bw, double* max_bw, int* iflag) {
+  guard(iflag, __func__, [&] { run_stream(3, mean_bw, max_bw); *iflag = 0; });
+}
+
+// ---- extension hooks (phist_amd_ext.h) ----
+void phist_amd_sparseMat_set_format(int C, int sigma, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(C >= 0 && sigma >= 0, PHIST_INVALID_INPUT, "negative SELL parameters");
+    PHX_REQUIRE(C == 0 || (C & (C - 1)) == 0, PHIST_INVALID_INPUT, "SELL chunk height must be a power of two");
+    rt().sell_C = C;
+    rt().sell_sigma = sigma;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_grid_blocks(int nblocks, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(nblocks >= 0, PHIST_INVALID_INPUT, "negative grid size");
+    rt().grid_blocks = nblocks;
+    *iflag = 0;
+  });
+}
+void phist_amd_get_stream(void** stream, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    *stream = (void*)rt().stream;
+    *iflag = 0;
+  });
+}
+void phist_amd_synchronize(int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    sync_stream();
+    *iflag = 0;
+  });
+}
+void phist_amd_timer_start(int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_HIP(hipEventRecord(rt().ev_start, rt().stream));
+    *iflag = 0;
+  });
+}
+void phist_amd_timer_stop(float* ms, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_HIP(hipEventRecord(rt().ev_stop, rt().stream));
+    PHX_HIP(hipEventSynchronize(rt().ev_stop));
+    PHX_HIP(hipEventElapsedTime(ms, rt().ev_start, rt().ev_stop));
+    *iflag = 0;
+  });
+}
+void phist_amd_kernel_timer(int enable, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    rt().ktimer_on = enable != 0;
+    rt().ktimer_ms = 0.0;
+    rt().ktimer_count = 0;
+    *iflag = 0;
+  });
+}
+void phist_amd_kernel_timer_read(double* total_ms, int64_t* count, int* iflag) {
+  guard(iflag, __func__, [&] {
+    *total_ms = rt().ktimer_ms;
+    *count = rt().ktimer_count;
+    *iflag = 0;
+  });
+}
+void phist_amd_device_info(char* name, int name_len, int* num_cu, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    hipDeviceProp_t prop;
+    PHX_HIP(hipGetDeviceProperties(&prop, rt().device));
+    if (name && name_len > 0) {
+      snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (num_cu) *num_cu = prop.multiProcessorCount;
+    *iflag = 0;
+  });
+}
+
+}  // extern "C"

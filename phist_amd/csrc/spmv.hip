@@ -1,0 +1,275 @@
+// spmv.hip -- SELL-C-sigma sparse matrix times row-major block vector.
+//
+// Computes, for every matrix row i and block column c (reference order of
+// spmvm_kernels.F90:261-271):
+//     tmp = shift_c * x(i,c);  tmp += val_j * x(col_j, c)  (row entries in
+//     ascending column order);  y(i,c) = alpha*tmp + beta*y(i,c)
+// with beta == 0 meaning y is not read (the non-temporal builtin path,
+// spmvm_kernels_nt.c:103-126), plus optional epilogues fused into the same
+// pass: column dots <y_new, y_new> and <x, y_new> (fused_spmv_mvdot,
+// phist_kernels_fused_decl.h:56) and U = gamma*y_new + delta*U
+// (fused_spmv_mvdot_mvadd, :77).
+//
+// Mapping onto CDNA4: a row is owned by LPR lanes of one wave64, each lane
+// holding VPL consecutive block columns (16-byte vectors where the layout
+// allows), so a wave covers 64/LPR consecutive SELL rows and the x gather of
+// one nonzero is one contiguous LPR*VPL*sizeof(T) segment.  val/col are read
+// coalesced from the column-major chunk layout with non-temporal loads, so
+// the L2 keeps the x rows being gathered.  The grid is persistent and
+// XCD-aware (amd_device.hpp).  Dots are reduced per wave with shuffles, per
+// workgroup through LDS, and across workgroups by a second fixed-order pass.
+#include "amd_device.hpp"
+#include <algorithm>
+
+namespace phx {
+
+template <typename T>
+struct SpmvArgs {
+  const int64_t* __restrict__ chunk_ptr;
+  const int32_t* __restrict__ chunk_w;
+  const int32_t* __restrict__ perm;
+  const int32_t* __restrict__ col;
+  const T* __restrict__ val;
+  int clog2;
+  int64_t nslots;
+  int64_t ntiles;
+  const T* __restrict__ x;
+  int64_t ldx;
+  T* __restrict__ y;
+  int64_t ldy;
+  T* __restrict__ u;
+  int64_t ldu;
+  int ncols;  // block columns handled by this launch
+  T alpha, beta, gamma, delta;
+  int flags;
+  T* __restrict__ partials;  // [gridDim.x][2][ncols]
+  T shift[64 * 4];
+};
+
+enum : int { F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16 };
+
+template <typename T, int VPL, int LPR>
+__global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
+  constexpr int RPW = 64 / LPR;
+  constexpr int RPB = (kBlock / 64) * RPW;
+  constexpr int UNR = 4;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int q = lane % LPR;
+  const int rw = lane / LPR;
+  const int c0 = q * VPL;
+  const bool colok = c0 < a.ncols;
+  const int C = 1 << a.clog2;
+  const int flags = a.flags;
+
+  T sh[VPL];
+#pragma unroll
+  for (int e = 0; e < VPL; e++) sh[e] = (colok && (flags & F_SHIFT)) ? a.shift[c0 + e] : T(0);
+  T dww[VPL], dvw[VPL];
+#pragma unroll
+  for (int e = 0; e < VPL; e++) dww[e] = dvw[e] = T(0);
+
+  const TileWalk tw = xcd_tiles(a.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t s = t * RPB + wv * RPW + rw;
+    if (s >= a.nslots || !colok) continue;
+    const int32_t row = a.perm[s];
+    if (row < 0) continue;
+    const int64_t ch = s >> a.clog2;
+    const int r = (int)(s & (C - 1));
+    const int w = a.chunk_w[ch];
+    const int64_t base = a.chunk_ptr[ch] + r;
+    const T* __restrict__ xb = a.x + c0;
+
+    T acc[VPL], xo[VPL];
+    if (flags & (F_SHIFT | F_DVW)) vload<T, VPL>(xb + (int64_t)row * a.ldx, xo);
+#pragma unroll
+    for (int e = 0; e < VPL; e++) acc[e] = (flags & F_SHIFT) ? sh[e] * xo[e] : T(0);
+
+    int j = 0;
+    for (; j + UNR <= w; j += UNR) {
+      T v[UNR];
+      int32_t cj[UNR];
+#pragma unroll
+      for (int k = 0; k < UNR; k++) {
+        const int64_t slot = base + (int64_t)(j + k) * C;
+        v[k] = __builtin_nontemporal_load(a.val + slot);
+        cj[k] = __builtin_nontemporal_load(a.col + slot);
+      }
+      T xg[UNR][VPL];
+#pragma unroll
+      for (int k = 0; k < UNR; k++) vload<T, VPL>(xb + (int64_t)cj[k] * a.ldx, xg[k]);
+#pragma unroll
+      for (int k = 0; k < UNR; k++)
+#pragma unroll
+        for (int e = 0; e < VPL; e++) acc[e] = fma(v[k], xg[k][e], acc[e]);
+    }
+    for (; j < w; j++) {
+      const int64_t slot = base + (int64_t)j * C;
+      const T v = __builtin_nontemporal_load(a.val + slot);
+      const int32_t cj = __builtin_nontemporal_load(a.col + slot);
+      T xg[VPL];
+      vload<T, VPL>(xb + (int64_t)cj * a.ldx, xg);
+#pragma unroll
+      for (int e = 0; e < VPL; e++) acc[e] = fma(v, xg[e], acc[e]);
+    }
+
+    T yn[VPL];
+    T* yp = a.y + (int64_t)row * a.ldy + c0;
+    if (flags & F_BETA) {
+      T yo[VPL];
+      vload<T, VPL>(yp, yo);
+#pragma unroll
+      for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e] + a.beta * yo[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e];
+    }
+    vstore<T, VPL>(yp, yn);
+    if (flags & F_DWW) {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) dww[e] = fma(yn[e], yn[e], dww[e]);
+    }
+    if (flags & F_DVW) {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) dvw[e] = fma(xo[e], yn[e], dvw[e]);
+    }
+    if (flags & F_AXPY) {
+      T* up = a.u + (int64_t)row * a.ldu + c0;
+      T uo[VPL], un[VPL];
+      vload<T, VPL>(up, uo);
+#pragma unroll
+      for (int e = 0; e < VPL; e++) un[e] = a.gamma * yn[e] + a.delta * uo[e];
+      vstore<T, VPL>(up, un);
+    }
+  }
+
+  if (flags & (F_DWW | F_DVW)) {
+    __shared__ T red[2][kBlock / 64][64 * VPL];
+#pragma unroll
+    for (int e = 0; e < VPL; e++) {
+      dww[e] = wave_sum_strided<LPR>(dww[e]);
+      dvw[e] = wave_sum_strided<LPR>(dvw[e]);
+    }
+    if (lane < LPR) {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) {
+        red[0][wv][lane * VPL + e] = dww[e];
+        red[1][wv][lane * VPL + e] = dvw[e];
+      }
+    }
+    __syncthreads();
+    const int width = a.ncols;
+    for (int i = threadIdx.x; i < 2 * width; i += kBlock) {
+      const int which = i / width, c = i % width;
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < kBlock / 64; k++) s += red[which][k][c];
+      a.partials[(int64_t)blockIdx.x * 2 * width + i] = s;
+    }
+  }
+}
+
+namespace {
+
+template <typename T, int VPL, int LPR>
+int launch_one(SpmvArgs<T>& a) {
+  auto kern = k_sell_spmv<T, VPL, LPR>;
+  constexpr int RPB = (kBlock / 64) * (64 / LPR);
+  a.ntiles = (a.nslots + RPB - 1) / RPB;
+  const int grid = persistent_blocks(kern, 0, a.ntiles);
+  if (a.flags & (F_DWW | F_DVW))
+    a.partials = (T*)device_work(sizeof(T) * ((size_t)grid * 2 * a.ncols + 2 * a.ncols));
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, a);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+  return grid;
+}
+
+template <typename T, int VPL>
+int launch_lpr(SpmvArgs<T>& a, int lpr) {
+  switch (lpr) {
+    case 1: return launch_one<T, VPL, 1>(a);
+    case 2: return launch_one<T, VPL, 2>(a);
+    case 4: return launch_one<T, VPL, 4>(a);
+    case 8: return launch_one<T, VPL, 8>(a);
+    case 16: return launch_one<T, VPL, 16>(a);
+    case 32: return launch_one<T, VPL, 32>(a);
+    default: return launch_one<T, VPL, 64>(a);
+  }
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+}  // namespace
+
+template <typename T>
+void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mvec<T>* x, T beta, Mvec<T>* y,
+                 const SpmvRequest& req, T* dot_ww_host, T* dot_vw_host, T gamma, T delta, Mvec<T>* U) {
+  const int nvec = x->nvec;
+  PHX_REQUIRE(y->nvec == nvec, PHIST_FUNCTIONAL_ERROR, "x has %d columns, y has %d", nvec, y->nvec);
+  PHX_REQUIRE(x->nrows == A->ncols && y->nrows == A->nrows, PHIST_INVALID_INPUT,
+              "shape mismatch: A is %lld x %lld, x has %lld rows, y has %lld rows", (long long)A->nrows,
+              (long long)A->ncols, (long long)x->nrows, (long long)y->nrows);
+  PHX_REQUIRE(!req.axpy_u || (U && U->nvec == nvec && U->nrows == y->nrows), PHIST_INVALID_INPUT, "bad U block");
+  if (x->root == y->root)
+    PHX_REQUIRE(x->jofs + nvec <= y->jofs || y->jofs + nvec <= x->jofs, PHIST_INVALID_INPUT,
+                "x and y overlap in memory");
+
+  constexpr int V16 = 16 / sizeof(T);
+  bool vec_ok = (nvec % V16 == 0) && (x->lda % V16 == 0) && (y->lda % V16 == 0) && aligned16(x->dev()) &&
+                aligned16(y->dev());
+  if (req.axpy_u) vec_ok = vec_ok && (U->lda % V16 == 0) && aligned16(U->dev());
+  const int VPL = vec_ok ? V16 : 1;
+  const int maxcols = 64 * VPL;
+
+  SpmvArgs<T> a;
+  a.chunk_ptr = A->d_chunk_ptr;
+  a.chunk_w = A->d_chunk_w;
+  a.perm = A->d_perm;
+  a.col = A->d_col;
+  a.val = A->d_val;
+  int clog2 = 0;
+  while ((1 << clog2) < A->C) clog2++;
+  a.clog2 = clog2;
+  a.nslots = A->nchunks * A->C;
+  a.ldx = x->lda;
+  a.ldy = y->lda;
+  a.ldu = U ? U->lda : 0;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.gamma = gamma;
+  a.delta = delta;
+  a.flags = (req.has_shift ? F_SHIFT : 0) | (req.has_beta ? F_BETA : 0) | (req.dot_ww ? F_DWW : 0) |
+            (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0);
+  const bool dots = req.dot_ww || req.dot_vw;
+
+  for (int cb = 0; cb < nvec; cb += maxcols) {
+    const int nc = std::min(maxcols, nvec - cb);
+    a.ncols = nc;
+    a.x = x->dev() + cb;
+    a.y = y->dev() + cb;
+    a.u = U ? U->dev() + cb : nullptr;
+    for (int c = 0; c < nc; c++) a.shift[c] = req.has_shift ? shifts_host[cb + c] : T(0);
+    const int groups = (nc + VPL - 1) / VPL;
+    int lpr = 1;
+    while (lpr < groups) lpr <<= 1;
+    const int grid = VPL == 1 ? launch_lpr<T, 1>(a, lpr) : launch_lpr<T, V16>(a, lpr);
+    if (dots) {
+      std::vector<T> sums(2 * nc);
+      reduce_partials_to_host(a.partials, grid, 2 * nc, sums.data());
+      for (int c = 0; c < nc; c++) {
+        if (dot_ww_host) dot_ww_host[cb + c] = sums[c];
+        if (dot_vw_host) dot_vw_host[cb + c] = sums[nc + c];
+      }
+    }
+  }
+}
+
+template void spmv_launch<double>(const SparseMat<double>*, double, const double*, const Mvec<double>*, double,
+                                  Mvec<double>*, const SpmvRequest&, double*, double*, double, double, Mvec<double>*);
+template void spmv_launch<float>(const SparseMat<float>*, float, const float*, const Mvec<float>*, float, Mvec<float>*,
+                                 const SpmvRequest&, float*, float*, float, float, Mvec<float>*);
+
+}  // namespace phx

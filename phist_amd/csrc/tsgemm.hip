@@ -1,0 +1,453 @@
+// tsgemm.hip -- tall-skinny products of block vectors with small dense matrices.
+//
+//   gemm_gram         C = alpha*V'W + beta*C        mvecT_times_mvec
+//                     (gemm_kernels_sC.F90:248-265, mvec_module.F90:2421-2743)
+//   gemm_times_sd     W = alpha*V*C + beta*W        mvec_times_sdMat
+//                     (gemm_kernels_sB.F90:15-43: W = beta*W, then
+//                     W += (alpha*C(j,:))*V(:,j) for j = 1..k)
+//   gemm_times_sd_inplace   V(:,1:m) = V*M          (gemm_kernels_sB.F90:595-612)
+//   gemm_times_sd_add_times_sd  W = W*D + V*C       (gemm_kernels_sB_add_sD.F90)
+//   fused_mvsdi_mvTmv W = W*C, then D = alpha*V'W + beta*D  (gemm_kernels_fused_sCD.F90)
+//   fused_mvsd_mvTmv  W = alpha*V*C + beta*W, then WtW = W'W
+//                     (gemm_kernels_sB_augmented.F90:149-195)
+//
+// Gram on the VALU: a row of the wider operand is split over LPR lanes (VPL
+// columns each, 16-byte loads), the narrower operand's row is broadcast
+// inside the lane group with cross-lane shuffles, every lane accumulates a
+// VPL x KB register tile; the tile sums go through the fixed-order two-stage
+// reduction and a finalize kernel forms alpha*S + beta*C on the device copy.
+// Row-mixing products keep alpha*C (and D) in LDS; each lane produces VPL
+// output columns of one row and reads the whole input row, so in-place
+// updates are safe: all lanes of a row sit in one wave and every store
+// depends on every load of that row.
+#include "amd_device.hpp"
+#include <algorithm>
+
+namespace phx {
+
+// ---------------------------------------------------------------------------
+// Gram products
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GramArgs {
+  int64_t nrows, ntiles;
+  const T* __restrict__ a;  // sliced operand (width na)
+  int64_t lda;
+  int na;
+  const T* __restrict__ b;  // broadcast operand (width nb <= LPR*VPL)
+  int64_t ldb;
+  int nb;
+  T* __restrict__ partials;  // [grid*4 waves][na*nb], index c*na + r (r over a, c over b)
+};
+
+template <typename T, int VPL, int LPR, int KB>
+__global__ void __launch_bounds__(kBlock) k_gram(const GramArgs<T> g) {
+  constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = lane % LPR, rw = lane / LPR, c0 = q * VPL;
+  const int grp = lane - q;
+  T acc[VPL][KB];
+#pragma unroll
+  for (int e = 0; e < VPL; e++)
+#pragma unroll
+    for (int c = 0; c < KB; c++) acc[e][c] = T(0);
+
+  const TileWalk tw = xcd_tiles(g.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t i = t * RPB + wv * RPW + rw;
+    const bool rowok = i < g.nrows;  // all lanes stay for the shuffles
+    T av[VPL], bv[VPL];
+#pragma unroll
+    for (int e = 0; e < VPL; e++) av[e] = bv[e] = T(0);
+    if (rowok && c0 < g.na) vload<T, VPL>(g.a + i * g.lda + c0, av);
+    if (rowok && c0 < g.nb) vload<T, VPL>(g.b + i * g.ldb + c0, bv);
+#pragma unroll
+    for (int s = 0; s < LPR; s++) {
+      if (s * VPL >= KB) break;
+#pragma unroll
+      for (int f = 0; f < VPL; f++) {
+        const int c = s * VPL + f;
+        if (c >= KB) break;
+        const T w = __shfl(bv[f], grp + s, 64);
+#pragma unroll
+        for (int e = 0; e < VPL; e++) acc[e][c] = fma(av[e], w, acc[e][c]);
+      }
+    }
+  }
+
+  // reduce over the rows of the wave; one partial tile per wave
+  const int nout = g.na * g.nb;
+  T* part = g.partials + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * nout;
+#pragma unroll
+  for (int e = 0; e < VPL; e++)
+#pragma unroll
+    for (int c = 0; c < KB; c++) {
+      const T s = wave_sum_strided<LPR>(acc[e][c]);
+      if (lane < LPR && c0 + e < g.na && c < g.nb) part[c * g.na + c0 + e] = s;
+    }
+}
+
+// C(r,c) = alpha * S + beta * C with S summed over the block partials;
+// S is stored c*na + r, transposed when the operands were swapped
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gram_finalize(const T* __restrict__ partials, int nblocks, int na, int nb,
+                                                          bool transposed, T alpha, T beta, T* __restrict__ C,
+                                                          int ldc) {
+  const int nout = na * nb;
+  for (int o = blockIdx.x * kBlock + threadIdx.x; o < nout; o += gridDim.x * kBlock) {
+    T s = T(0);
+    for (int b = 0; b < nblocks; b++) s += partials[(int64_t)b * nout + o];
+    const int r = o % na, c = o / na;
+    const int ci = transposed ? r : c, ri = transposed ? c : r;
+    T* cp = C + (int64_t)ci * ldc + ri;
+    *cp = beta == T(0) ? alpha * s : alpha * s + beta * *cp;
+  }
+}
+
+namespace {
+
+inline bool al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+template <typename T, int VPL, int LPR, int KB>
+int gram_launch_one(GramArgs<T>& g) {
+  auto kern = k_gram<T, VPL, LPR, KB>;
+  constexpr int RPB = (kBlock / 64) * (64 / LPR);
+  g.ntiles = (g.nrows + RPB - 1) / RPB;
+  const int grid = persistent_blocks(kern, 0, g.ntiles);
+  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * g.na * g.nb);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, g);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+  return grid * (kBlock / 64);
+}
+
+template <typename T, int VPL, int KB>
+int gram_lpr(GramArgs<T>& g, int lpr) {
+  switch (lpr) {
+    case 1: return gram_launch_one<T, VPL, 1, KB>(g);
+    case 2: return gram_launch_one<T, VPL, 2, KB>(g);
+    case 4: return gram_launch_one<T, VPL, 4, KB>(g);
+    case 8: return gram_launch_one<T, VPL, 8, KB>(g);
+    case 16: return gram_launch_one<T, VPL, 16, KB>(g);
+    default: return gram_launch_one<T, VPL, 32, KB>(g);
+  }
+}
+
+}  // namespace
+
+// Gram of the column blocks a (na cols) and b (nb cols): S = a'b, added into
+// C(roff:, coff:) as C = alpha*S + beta*C (or its transpose)
+template <typename T>
+static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int64_t ldb, int nb, int64_t nrows,
+                       bool transposed, T beta, T* Cdev, int ldc) {
+  constexpr int V16 = 16 / sizeof(T);
+  const bool vec = (na % V16 == 0) && (nb % V16 == 0) && (lda % V16 == 0) && (ldb % V16 == 0) && al16(a) && al16(b);
+  const int VPL = vec ? V16 : 1;
+  GramArgs<T> g;
+  g.nrows = nrows;
+  g.a = a;
+  g.lda = lda;
+  g.na = na;
+  g.b = b;
+  g.ldb = ldb;
+  g.nb = nb;
+  const int groups = (na + VPL - 1) / VPL;
+  int lpr = 1;
+  while (lpr < groups) lpr <<= 1;
+  int nparts;  // partial tiles written (one per wave)
+  if (nb <= 8) {
+    nparts = VPL == 1 ? gram_lpr<T, 1, 8>(g, lpr) : gram_lpr<T, V16, 8>(g, lpr);
+  } else {
+    nparts = VPL == 1 ? gram_lpr<T, 1, 32>(g, lpr) : gram_lpr<T, V16, 32>(g, lpr);
+  }
+  const int nout = na * nb;
+  const int fblocks = std::max(1, std::min(64, (nout + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, na, nb,
+                     transposed, alpha, beta, Cdev, ldc);
+  PHX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C) {
+  const int m = V->nvec, k = W->nvec;
+  PHX_REQUIRE(V->nrows == W->nrows, PHIST_INVALID_INPUT, "V and W have different lengths");
+  PHX_REQUIRE(C->nrows == m && C->ncols == k, PHIST_INVALID_INPUT, "C is %dx%d, V'W is %dx%d", C->nrows, C->ncols, m,
+              k);
+  const int ldc = C->root->ld;
+  // the sliced operand is the wider one; the broadcast side is cut in blocks
+  // of at most 32 columns (each block one pass over the rows)
+  const bool swap = k > m;
+  const Mvec<T>* A = swap ? W : V;
+  const Mvec<T>* B = swap ? V : W;
+  const int na = A->nvec, nbt = B->nvec;
+  constexpr int V16 = 16 / sizeof(T);
+  const int amax = 64 * V16;  // lanes of one wave
+  for (int a0 = 0; a0 < na; a0 += amax) {
+    const int nac = std::min(amax, na - a0);
+    for (int b0 = 0; b0 < nbt; b0 += 32) {
+      const int nbc = std::min(32, nbt - b0);
+      // output block: rows a0.., cols b0.. of S (or transposed)
+      T* cblk = swap ? C->dev() + (int64_t)a0 * ldc + b0 : C->dev() + (int64_t)b0 * ldc + a0;
+      if (nbc > nac) {
+        // broadcast side wider than the sliced block: swap roles for this block
+        gram_block<T>(alpha, B->dev() + b0, B->lda, nbc, A->dev() + a0, A->lda, nac, A->nrows, !swap, beta, cblk, ldc);
+      } else {
+        gram_block<T>(alpha, A->dev() + a0, A->lda, nac, B->dev() + b0, B->lda, nbc, A->nrows, swap, beta, cblk, ldc);
+      }
+    }
+  }
+  sd_from_device(C);
+}
+
+// ---------------------------------------------------------------------------
+// row-mixing products: W(i,:) = f(W(i,:)) + sum_j alpha*C(j,:) * V(i,j)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct MixArgs {
+  int64_t nrows, ntiles;
+  const T* v;
+  int64_t ldv;
+  int k;
+  T* w;
+  int64_t ldw;
+  int m;
+  const T* cdev;
+  int ldc;
+  const T* ddev;
+  int ldd;
+  T alpha, beta;
+  int mode;  // 0: W = alpha*V*C + beta*W, 1: W = W*D + V*C, 2: W = V*C (W may alias V)
+  int vvec;  // V and W rows may be read with 16-byte vectors
+};
+
+template <typename T, int VPL, int LPR>
+__global__ void __launch_bounds__(kBlock) k_rowmix(const MixArgs<T> p) {
+  extern __shared__ char smem_raw[];
+  T* sC = reinterpret_cast<T*>(smem_raw);  // [k][m] alpha*C(j,c) at j*m + c
+  T* sD = sC + p.k * p.m;                  // [m][m] D(j,c) at j*m + c
+  for (int o = threadIdx.x; o < p.k * p.m; o += kBlock) {
+    const int j = o / p.m, c = o % p.m;
+    sC[o] = p.alpha * p.cdev[(int64_t)c * p.ldc + j];
+  }
+  if (p.mode == 1)
+    for (int o = threadIdx.x; o < p.m * p.m; o += kBlock) {
+      const int j = o / p.m, c = o % p.m;
+      sD[o] = p.ddev[(int64_t)c * p.ldd + j];
+    }
+  __syncthreads();
+  constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
+  constexpr int V16 = 16 / sizeof(T);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = lane % LPR, rw = lane / LPR, c0 = q * VPL;
+  if (c0 >= p.m) return;
+  const TileWalk tw = xcd_tiles(p.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t i = t * RPB + wv * RPW + rw;
+    if (i >= p.nrows) continue;
+    T acc[VPL];
+    T* wp = p.w + i * p.ldw;
+    if (p.mode == 0 && p.beta != T(0)) {
+      T wo[VPL];
+      vload<T, VPL>(wp + c0, wo);
+#pragma unroll
+      for (int e = 0; e < VPL; e++) acc[e] = p.beta * wo[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) acc[e] = T(0);
+    }
+    if (p.mode == 1) {  // acc = W(i,:) * D(:, c0..)
+      int j = 0;
+      if (p.vvec)
+        for (; j + V16 <= p.m; j += V16) {
+          T wr[V16];
+          vload<T, V16>(wp + j, wr);
+#pragma unroll
+          for (int u = 0; u < V16; u++)
+#pragma unroll
+            for (int e = 0; e < VPL; e++) acc[e] = fma(wr[u], sD[(j + u) * p.m + c0 + e], acc[e]);
+        }
+      for (; j < p.m; j++) {
+        const T wr = wp[j];
+#pragma unroll
+        for (int e = 0; e < VPL; e++) acc[e] = fma(wr, sD[j * p.m + c0 + e], acc[e]);
+      }
+    }
+    const T* vp = p.v + i * p.ldv;
+    int j = 0;
+    if (p.vvec)
+      for (; j + V16 <= p.k; j += V16) {
+        T vr[V16];
+        vload<T, V16>(vp + j, vr);
+#pragma unroll
+        for (int u = 0; u < V16; u++)
+#pragma unroll
+          for (int e = 0; e < VPL; e++) acc[e] = fma(sC[(j + u) * p.m + c0 + e], vr[u], acc[e]);
+      }
+    for (; j < p.k; j++) {
+      const T vr = vp[j];
+#pragma unroll
+      for (int e = 0; e < VPL; e++) acc[e] = fma(sC[j * p.m + c0 + e], vr, acc[e]);
+    }
+    vstore<T, VPL>(wp + c0, acc);
+  }
+}
+
+namespace {
+
+template <typename T, int VPL, int LPR>
+void mix_launch_one(MixArgs<T>& p, size_t lds) {
+  auto kern = k_rowmix<T, VPL, LPR>;
+  constexpr int RPB = (kBlock / 64) * (64 / LPR);
+  p.ntiles = (p.nrows + RPB - 1) / RPB;
+  const int grid = persistent_blocks(kern, lds, p.ntiles);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, p);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+}
+
+template <typename T, int VPL>
+void mix_lpr(MixArgs<T>& p, int lpr, size_t lds) {
+  switch (lpr) {
+    case 1: return mix_launch_one<T, VPL, 1>(p, lds);
+    case 2: return mix_launch_one<T, VPL, 2>(p, lds);
+    case 4: return mix_launch_one<T, VPL, 4>(p, lds);
+    case 8: return mix_launch_one<T, VPL, 8>(p, lds);
+    case 16: return mix_launch_one<T, VPL, 16>(p, lds);
+    case 32: return mix_launch_one<T, VPL, 32>(p, lds);
+    default: return mix_launch_one<T, VPL, 64>(p, lds);
+  }
+}
+
+constexpr size_t kMixLds = 64 * 1024;
+
+// one launch; the caller guarantees the LDS footprint fits and m <= 64*VPL
+template <typename T>
+void rowmix(int mode, T alpha, const T* v, int64_t ldv, int k, const T* cdev, int ldc, T beta, T* w, int64_t ldw,
+            int m, const T* ddev, int ldd, int64_t nrows) {
+  constexpr int V16 = 16 / sizeof(T);
+  const bool vvec = (ldv % V16 == 0) && al16(v) && (mode != 1 || ((ldw % V16 == 0) && al16(w)));
+  const bool ovec = (m % V16 == 0) && (ldw % V16 == 0) && al16(w);
+  const int VPL = ovec ? V16 : 1;
+  MixArgs<T> p;
+  p.nrows = nrows;
+  p.v = v;
+  p.ldv = ldv;
+  p.k = k;
+  p.w = w;
+  p.ldw = ldw;
+  p.m = m;
+  p.cdev = cdev;
+  p.ldc = ldc;
+  p.ddev = ddev;
+  p.ldd = ldd;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.mode = mode;
+  p.vvec = vvec ? 1 : 0;
+  const size_t lds = sizeof(T) * ((size_t)k * m + (mode == 1 ? (size_t)m * m : 0));
+  const int groups = (m + VPL - 1) / VPL;
+  int lpr = 1;
+  while (lpr < groups) lpr <<= 1;
+  if (VPL == 1)
+    mix_lpr<T, 1>(p, lpr, lds);
+  else
+    mix_lpr<T, V16>(p, lpr, lds);
+}
+
+template <typename T>
+bool mix_fits(int k, int m, bool with_d) {
+  constexpr int V16 = 16 / sizeof(T);
+  return m <= 64 * V16 && sizeof(T) * ((size_t)k * m + (with_d ? (size_t)m * m : 0)) <= kMixLds;
+}
+
+}  // namespace
+
+template <typename T>
+void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W) {
+  const int k = V->nvec, m = W->nvec;
+  PHX_REQUIRE(V->nrows == W->nrows, PHIST_INVALID_INPUT, "V and W have different lengths");
+  PHX_REQUIRE(C->nrows == k && C->ncols == m, PHIST_INVALID_INPUT, "C is %dx%d, expected %dx%d", C->nrows, C->ncols, k,
+              m);
+  if (V->root == W->root)
+    PHX_REQUIRE(V->jofs + k <= W->jofs || W->jofs + m <= V->jofs, PHIST_INVALID_INPUT, "V and W overlap in memory");
+  const int ldc = C->root->ld;
+  // output columns in blocks; k in blocks (later blocks accumulate with beta=1)
+  constexpr int V16 = 16 / sizeof(T);
+  const int mb = std::min(m, 64 * V16);
+  for (int c0 = 0; c0 < m; c0 += mb) {
+    const int mc = std::min(mb, m - c0);
+    int kb = std::max(1, (int)(kMixLds / sizeof(T)) / std::max(1, mc));
+    for (int j0 = 0; j0 < k || (k == 0 && j0 == 0); j0 += kb) {
+      const int kc = std::min(kb, k - j0);
+      const T b = j0 == 0 ? beta : T(1);
+      rowmix<T>(0, alpha, V->dev() + j0, V->lda, kc, C->dev() + (int64_t)c0 * ldc + j0, ldc, b, W->dev() + c0, W->lda,
+                mc, nullptr, 0, V->nrows);
+      if (k == 0) break;
+    }
+  }
+}
+
+template <typename T>
+void gemm_times_sd_inplace(Mvec<T>* V, const SdMat<T>* M) {
+  const int k = V->nvec, m = M->ncols;
+  PHX_REQUIRE(M->nrows == k && m <= k, PHIST_INVALID_INPUT, "M is %dx%d, V has %d columns", M->nrows, M->ncols, k);
+  if (mix_fits<T>(k, m, false)) {
+    rowmix<T>(2, T(1), V->dev(), V->lda, k, M->dev(), M->root->ld, T(0), V->dev(), V->lda, m, nullptr, 0, V->nrows);
+    return;
+  }
+  // too large for one pass: go through a copy of V
+  Mvec<T> tmp;
+  tmp.map = V->map;
+  tmp.nrows = V->nrows;
+  tmp.nvec = k;
+  tmp.lda = k;
+  PHX_HIP(hipMallocAsync((void**)&tmp.dev_base, sizeof(T) * std::max<int64_t>(1, V->nrows * k), rt().stream));
+  blas_copy<T>(V, &tmp);
+  Mvec<T> out = *V;
+  out.root = V->root;
+  out.nvec = m;
+  out.jofs = V->jofs;
+  gemm_times_sd<T>(T(1), &tmp, M, T(0), &out);
+  PHX_HIP(hipFreeAsync(tmp.dev_base, rt().stream));
+}
+
+template <typename T>
+void gemm_times_sd_add_times_sd(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W, const SdMat<T>* D) {
+  const int k = V->nvec, m = W->nvec;
+  PHX_REQUIRE(V->nrows == W->nrows, PHIST_INVALID_INPUT, "V and W have different lengths");
+  PHX_REQUIRE(C->nrows == k && C->ncols == m && D->nrows == m && D->ncols == m, PHIST_INVALID_INPUT,
+              "sdMat shapes do not match V (%d cols) and W (%d cols)", k, m);
+  PHX_REQUIRE(mix_fits<T>(k, m, true), PHIST_NOT_IMPLEMENTED, "W*D + V*C with %d x %d blocks exceeds one pass", k, m);
+  rowmix<T>(1, T(1), V->dev(), V->lda, k, C->dev(), C->root->ld, T(0), W->dev(), W->lda, m, D->dev(), D->root->ld,
+            V->nrows);
+}
+
+template <typename T>
+void gemm_fused_mvsdi_mvTmv(T alpha, const Mvec<T>* V, Mvec<T>* W, const SdMat<T>* C, T beta, SdMat<T>* D) {
+  gemm_times_sd_inplace<T>(W, C);
+  Mvec<T> wv = *W;
+  wv.root = W->root;
+  wv.nvec = C->ncols;
+  gemm_gram<T>(alpha, V, &wv, beta, D);
+}
+
+template <typename T>
+void gemm_fused_mvsd_mvTmv(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W, SdMat<T>* WtW) {
+  gemm_times_sd<T>(alpha, V, C, beta, W);
+  gemm_gram<T>(T(1), W, W, T(0), WtW);
+}
+
+#define PHX_INST(T)                                                                                  \
+  template void gemm_gram<T>(T, const Mvec<T>*, const Mvec<T>*, T, SdMat<T>*);                       \
+  template void gemm_times_sd<T>(T, const Mvec<T>*, const SdMat<T>*, T, Mvec<T>*);                   \
+  template void gemm_times_sd_inplace<T>(Mvec<T>*, const SdMat<T>*);                                 \
+  template void gemm_times_sd_add_times_sd<T>(const Mvec<T>*, const SdMat<T>*, Mvec<T>*, const SdMat<T>*); \
+  template void gemm_fused_mvsdi_mvTmv<T>(T, const Mvec<T>*, Mvec<T>*, const SdMat<T>*, T, SdMat<T>*); \
+  template void gemm_fused_mvsd_mvTmv<T>(T, const Mvec<T>*, const SdMat<T>*, T, Mvec<T>*, SdMat<T>*);
+PHX_INST(double)
+PHX_INST(float)
+#undef PHX_INST
+
+}  // namespace phx

@@ -1,0 +1,198 @@
+"""ctypes access to the oracle (oracle/liboracle.so, the C restatement of the
+builtin kernels) and to the reference's own leaf kernels compiled by
+oracle/Makefile into oracle/_ref/libbuiltin_leaf.so.  Test code only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libbuiltin_leaf.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+MATRICES = os.path.join(GOLDEN, "matrices")
+
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Kiss(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint64), ("y", ctypes.c_uint64), ("z", ctypes.c_uint64), ("c", ctypes.c_uint64)]
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(
+                os.path.join(ORACLE_DIR, "builtin_oracle.c")):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR, "all"])
+        _oracle = ctypes.CDLL(ORACLE_SO)
+        _oracle.oracle_kiss_raw.restype = ctypes.c_uint64
+        _oracle.oracle_kiss_double.restype = ctypes.c_double
+    return _oracle
+
+
+def ref_leaf():
+    """the flang-built builtin leaf kernels, or None if not built"""
+    if not os.path.exists(REF_SO):
+        return None
+    return ctypes.CDLL(REF_SO)
+
+
+class Crs:
+    """builtin CRS: 1-based int64 row_offset, 1-based int32 col_idx, values"""
+
+    def __init__(self, nrows, ncols, row_offset, col_idx, val):
+        self.nrows, self.ncols = nrows, ncols
+        self.row_offset, self.col_idx, self.val = row_offset, col_idx, val
+
+    @property
+    def nnz(self):
+        return int(self.row_offset[-1] - 1)
+
+    def dense(self):
+        A = np.zeros((self.nrows, self.ncols))
+        for i in range(self.nrows):
+            for k in range(self.row_offset[i] - 1, self.row_offset[i + 1] - 1):
+                A[i, self.col_idx[k] - 1] += self.val[k]
+        return A
+
+
+def crs_from_mm(path):
+    o = oracle()
+    nr, nc = ctypes.c_int64(), ctypes.c_int64()
+    ro, ci, va = c_i64p(), c_i32p(), c_dp()
+    rc = o.oracle_crs_from_mm(path.encode(), ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(ro), ctypes.byref(ci),
+                              ctypes.byref(va))
+    assert rc == 0, rc
+    n = nr.value
+    row_offset = np.ctypeslib.as_array(ro, (n + 1,)).copy()
+    nnz = int(row_offset[-1] - 1)
+    col = np.ctypeslib.as_array(ci, (max(1, nnz),))[:nnz].copy()
+    val = np.ctypeslib.as_array(va, (max(1, nnz),))[:nnz].copy()
+    for q in (ro, ci, va):
+        o.oracle_free(q)
+    return Crs(n, nc.value, row_offset, col, val)
+
+
+def crs_from_rows(nrows, ncols, rows):
+    """rows: list of (cols0based, vals) in fill order -> sorted builtin CRS"""
+    o = oracle()
+    ptr = np.zeros(nrows + 1, np.int64)
+    for i, (c, _) in enumerate(rows):
+        ptr[i + 1] = ptr[i] + len(c)
+    gcol = np.concatenate([np.asarray(c, np.int64) for c, _ in rows]) if ptr[-1] else np.zeros(1, np.int64)
+    gval = np.concatenate([np.asarray(v, np.float64) for _, v in rows]) if ptr[-1] else np.zeros(1)
+    ro = np.empty(nrows + 1, np.int64)
+    ci = np.empty(max(1, ptr[-1]), np.int32)
+    va = np.empty(max(1, ptr[-1]))
+    o.oracle_crs_sort_rows(ctypes.c_int64(nrows), p(ptr), p(gcol), p(gval), p(ro), p(ci), p(va))
+    return Crs(nrows, ncols, ro, ci[:ptr[-1]], va[:ptr[-1]])
+
+
+def spmv(A, alpha, x, beta, y, shifts=None, nt=False):
+    """returns alpha*(A + diag(shifts))x + beta*y (builtin FP order)"""
+    o = oracle()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.array(y, np.float64, copy=True, order="C")
+    nvec = x.shape[1]
+    sh = None if shifts is None else np.ascontiguousarray(shifts, np.float64)
+    o.oracle_spmv(ctypes.c_int64(A.nrows), p(A.row_offset), p(A.col_idx), p(A.val), ctypes.c_int(nvec),
+                  ctypes.c_double(alpha), None if sh is None else p(sh), p(x), ctypes.c_int64(nvec),
+                  ctypes.c_double(beta), p(y), ctypes.c_int64(nvec), ctypes.c_int(1 if nt else 0))
+    return y
+
+
+def dot(v, w):
+    o = oracle()
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.empty(v.shape[1])
+    o.oracle_dot(ctypes.c_int64(v.shape[0]), ctypes.c_int(v.shape[1]), p(v), ctypes.c_int64(v.shape[1]), p(w),
+                 ctypes.c_int64(w.shape[1]), p(out))
+    return out
+
+
+def nrm2(v):
+    o = oracle()
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.empty(v.shape[1])
+    o.oracle_nrm2(ctypes.c_int64(v.shape[0]), ctypes.c_int(v.shape[1]), p(v), ctypes.c_int64(v.shape[1]), p(out))
+    return out
+
+
+def gram(alpha, v, w, beta, c):
+    o = oracle()
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    cf = np.asfortranarray(np.array(c, np.float64))
+    o.oracle_gram(ctypes.c_int64(v.shape[0]), ctypes.c_int(v.shape[1]), ctypes.c_int(w.shape[1]), ctypes.c_double(alpha),
+                  p(v), ctypes.c_int64(v.shape[1]), p(w), ctypes.c_int64(w.shape[1]), ctypes.c_double(beta), p(cf),
+                  ctypes.c_int(v.shape[1]))
+    return np.ascontiguousarray(cf)
+
+
+def times_sdmat(alpha, v, c, beta, w):
+    o = oracle()
+    v = np.ascontiguousarray(v, np.float64)
+    cf = np.asfortranarray(np.asarray(c, np.float64))
+    w = np.array(w, np.float64, copy=True, order="C")
+    k, m = cf.shape
+    o.oracle_times_sdmat(ctypes.c_int64(v.shape[0]), ctypes.c_int(k), ctypes.c_int(m), ctypes.c_double(alpha), p(v),
+                         ctypes.c_int64(v.shape[1]), p(cf), ctypes.c_int(k), ctypes.c_double(beta), p(w),
+                         ctypes.c_int64(w.shape[1]))
+    return w
+
+
+def times_sdmat_inplace(v, m):
+    o = oracle()
+    v = np.array(v, np.float64, copy=True, order="C")
+    mf = np.asfortranarray(np.asarray(m, np.float64))
+    k, mm = mf.shape
+    o.oracle_times_sdmat_inplace(ctypes.c_int64(v.shape[0]), ctypes.c_int(k), ctypes.c_int(mm), p(v),
+                                 ctypes.c_int64(v.shape[1]), p(mf), ctypes.c_int(k))
+    return v[:, :mm]
+
+
+def cholesky(m, rank_tol):
+    o = oracle()
+    a = np.asfortranarray(np.array(m, np.float64))
+    n = a.shape[0]
+    perm = np.zeros(n, np.int32)
+    rank = ctypes.c_int()
+    o.oracle_cholesky(p(a), ctypes.c_int(n), ctypes.c_int(n), p(perm), ctypes.byref(rank), ctypes.c_double(rank_tol))
+    return np.ascontiguousarray(a), perm, rank.value
+
+
+def kiss_doubles(n, skip_before=0):
+    o = oracle()
+    s = Kiss()
+    o.oracle_kiss_init(ctypes.byref(s))
+    if skip_before:
+        o.oracle_kiss_skip(ctypes.byref(s), ctypes.c_uint64(skip_before))
+    out = np.empty(n)
+    o.oracle_kiss_fill(ctypes.byref(s), ctypes.c_int64(n), p(out))
+    return out
+
+
+def mvec_random_sequence(shapes):
+    """values of consecutive mvec_random calls on one process (row-major)"""
+    o = oracle()
+    s = Kiss()
+    o.oracle_kiss_init(ctypes.byref(s))
+    outs = []
+    for (n, k) in shapes:
+        v = np.empty((n, k))
+        o.oracle_mvec_random(ctypes.byref(s), ctypes.c_int64(n), ctypes.c_int(k), p(v), ctypes.c_int64(k))
+        outs.append(v)
+    return outs
