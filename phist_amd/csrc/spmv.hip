@@ -213,6 +213,9 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
               "shape mismatch: A is %lld x %lld, x has %lld rows, y has %lld rows", (long long)A->nrows,
               (long long)A->ncols, (long long)x->nrows, (long long)y->nrows);
   PHX_REQUIRE(!req.axpy_u || (U && U->nvec == nvec && U->nrows == y->nrows), PHIST_INVALID_INPUT, "bad U block");
+  PHX_REQUIRE(!(req.has_shift || req.dot_vw) || A->nrows == A->ncols, PHIST_INVALID_INPUT,
+              "shifts and <x,y> dots need a square matrix (A is %lld x %lld)", (long long)A->nrows,
+              (long long)A->ncols);
   if (x->root == y->root)
     PHX_REQUIRE(x->jofs + nvec <= y->jofs || y->jofs + nvec <= x->jofs, PHIST_INVALID_INPUT,
                 "x and y overlap in memory");

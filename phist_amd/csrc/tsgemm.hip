@@ -136,36 +136,43 @@ int gram_lpr(GramArgs<T>& g, int lpr) {
 
 }  // namespace
 
-// Gram of the column blocks a (na cols) and b (nb cols): S = a'b, added into
-// C(roff:, coff:) as C = alpha*S + beta*C (or its transpose)
+// Gram of the column blocks a (na cols) and b (nb <= 32 cols, nb <= na):
+// S = a'b, written to C as C = alpha*S + beta*C (or its transpose); the a
+// side is cut into pieces one lane group covers (32 lanes of VPL columns)
 template <typename T>
 static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int64_t ldb, int nb, int64_t nrows,
                        bool transposed, T beta, T* Cdev, int ldc) {
   constexpr int V16 = 16 / sizeof(T);
   const bool vec = (na % V16 == 0) && (nb % V16 == 0) && (lda % V16 == 0) && (ldb % V16 == 0) && al16(a) && al16(b);
   const int VPL = vec ? V16 : 1;
-  GramArgs<T> g;
-  g.nrows = nrows;
-  g.a = a;
-  g.lda = lda;
-  g.na = na;
-  g.b = b;
-  g.ldb = ldb;
-  g.nb = nb;
-  const int groups = (na + VPL - 1) / VPL;
-  int lpr = 1;
-  while (lpr < groups) lpr <<= 1;
-  int nparts;  // partial tiles written (one per wave)
-  if (nb <= 8) {
-    nparts = VPL == 1 ? gram_lpr<T, 1, 8>(g, lpr) : gram_lpr<T, V16, 8>(g, lpr);
-  } else {
-    nparts = VPL == 1 ? gram_lpr<T, 1, 32>(g, lpr) : gram_lpr<T, V16, 32>(g, lpr);
+  const int apiece = 32 * VPL;
+  for (int p0 = 0; p0 < na; p0 += apiece) {
+    GramArgs<T> g;
+    g.nrows = nrows;
+    g.a = a + p0;
+    g.lda = lda;
+    g.na = std::min(apiece, na - p0);
+    g.b = b;
+    g.ldb = ldb;
+    g.nb = nb;
+    // the lane group must hold every column of both operands' rows
+    const int groups = (std::max(g.na, nb) + VPL - 1) / VPL;
+    int lpr = 1;
+    while (lpr < groups) lpr <<= 1;
+    int nparts;  // partial tiles written (one per wave)
+    if (nb <= 8) {
+      nparts = VPL == 1 ? gram_lpr<T, 1, 8>(g, lpr) : gram_lpr<T, V16, 8>(g, lpr);
+    } else {
+      nparts = VPL == 1 ? gram_lpr<T, 1, 32>(g, lpr) : gram_lpr<T, V16, 32>(g, lpr);
+    }
+    // S rows p0.. of this piece land in C rows (or columns, transposed) p0..
+    T* cpiece = transposed ? Cdev + (int64_t)p0 * ldc : Cdev + p0;
+    const int nout = g.na * nb;
+    const int fblocks = std::max(1, std::min(64, (nout + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, g.na, nb,
+                       transposed, alpha, beta, cpiece, ldc);
+    PHX_HIP(hipGetLastError());
   }
-  const int nout = na * nb;
-  const int fblocks = std::max(1, std::min(64, (nout + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, na, nb,
-                     transposed, alpha, beta, Cdev, ldc);
-  PHX_HIP(hipGetLastError());
 }
 
 template <typename T>
@@ -181,21 +188,11 @@ void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C)
   const Mvec<T>* A = swap ? W : V;
   const Mvec<T>* B = swap ? V : W;
   const int na = A->nvec, nbt = B->nvec;
-  constexpr int V16 = 16 / sizeof(T);
-  const int amax = 64 * V16;  // lanes of one wave
-  for (int a0 = 0; a0 < na; a0 += amax) {
-    const int nac = std::min(amax, na - a0);
-    for (int b0 = 0; b0 < nbt; b0 += 32) {
-      const int nbc = std::min(32, nbt - b0);
-      // output block: rows a0.., cols b0.. of S (or transposed)
-      T* cblk = swap ? C->dev() + (int64_t)a0 * ldc + b0 : C->dev() + (int64_t)b0 * ldc + a0;
-      if (nbc > nac) {
-        // broadcast side wider than the sliced block: swap roles for this block
-        gram_block<T>(alpha, B->dev() + b0, B->lda, nbc, A->dev() + a0, A->lda, nac, A->nrows, !swap, beta, cblk, ldc);
-      } else {
-        gram_block<T>(alpha, A->dev() + a0, A->lda, nac, B->dev() + b0, B->lda, nbc, A->nrows, swap, beta, cblk, ldc);
-      }
-    }
+  for (int b0 = 0; b0 < nbt; b0 += 32) {
+    const int nbc = std::min(32, nbt - b0);
+    // output block: all A columns x B columns b0..b0+nbc
+    T* cblk = swap ? C->dev() + b0 : C->dev() + (int64_t)b0 * ldc;
+    gram_block<T>(alpha, A->dev(), A->lda, na, B->dev() + b0, B->lda, nbc, A->nrows, swap, beta, cblk, ldc);
   }
   sd_from_device(C);
 }
@@ -320,7 +317,7 @@ void mix_lpr(MixArgs<T>& p, int lpr, size_t lds) {
   }
 }
 
-constexpr size_t kMixLds = 64 * 1024;
+constexpr size_t kMixLds = 32 * 1024;
 
 // one launch; the caller guarantees the LDS footprint fits and m <= 64*VPL
 template <typename T>
@@ -347,6 +344,8 @@ void rowmix(int mode, T alpha, const T* v, int64_t ldv, int k, const T* cdev, in
   p.mode = mode;
   p.vvec = vvec ? 1 : 0;
   const size_t lds = sizeof(T) * ((size_t)k * m + (mode == 1 ? (size_t)m * m : 0));
+  PHX_REQUIRE(m <= 64 * VPL && lds <= kMixLds, PHIST_FUNCTIONAL_ERROR, "row-mix launch of %d x %d exceeds one pass", k,
+              m);
   const int groups = (m + VPL - 1) / VPL;
   int lpr = 1;
   while (lpr < groups) lpr <<= 1;
@@ -356,10 +355,11 @@ void rowmix(int mode, T alpha, const T* v, int64_t ldv, int k, const T* cdev, in
     mix_lpr<T, V16>(p, lpr, lds);
 }
 
+// one launch: at most 64 output columns (64 lane groups of >= 1 column) and
+// the LDS copies of C (and D) within kMixLds
 template <typename T>
 bool mix_fits(int k, int m, bool with_d) {
-  constexpr int V16 = 16 / sizeof(T);
-  return m <= 64 * V16 && sizeof(T) * ((size_t)k * m + (with_d ? (size_t)m * m : 0)) <= kMixLds;
+  return m <= 64 && sizeof(T) * ((size_t)k * m + (with_d ? (size_t)m * m : 0)) <= kMixLds;
 }
 
 }  // namespace
@@ -373,9 +373,8 @@ void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>
   if (V->root == W->root)
     PHX_REQUIRE(V->jofs + k <= W->jofs || W->jofs + m <= V->jofs, PHIST_INVALID_INPUT, "V and W overlap in memory");
   const int ldc = C->root->ld;
-  // output columns in blocks; k in blocks (later blocks accumulate with beta=1)
-  constexpr int V16 = 16 / sizeof(T);
-  const int mb = std::min(m, 64 * V16);
+  // output columns in blocks of 64; k in blocks (later blocks accumulate with beta=1)
+  const int mb = std::min(m, 64);
   for (int c0 = 0; c0 < m; c0 += mb) {
     const int mc = std::min(mb, m - c0);
     int kb = std::max(1, (int)(kMixLds / sizeof(T)) / std::max(1, mc));
