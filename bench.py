@@ -117,6 +117,9 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run the library's STREAM copy kernel (1 GiB in, 1 GiB out per launch) so a "
+                         "counter profile of this command holds a known-size reference dispatch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -130,6 +133,11 @@ def main():
     iflag = ctypes.c_int(0)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
     N, nvec = args.rows, args.nvec
+    if args.copy_calibration:
+        mean_bw, max_bw = ctypes.c_double(), ctypes.c_double()
+        L.phist_bench_stream_copy(ctypes.byref(mean_bw), ctypes.byref(max_bw), ctypes.byref(iflag))
+        print(f"bench: STREAM copy {mean_bw.value:.0f} GB/s mean, {max_bw.value:.0f} GB/s max",
+              file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     A = be.banded_random(N, 32, 50000, 40.0, 2024)
     t_build = time.perf_counter() - t0

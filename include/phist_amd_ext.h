@@ -21,6 +21,9 @@ void phist_amd_sparseMat_set_format(int C, int sigma, int* iflag);
 /* upper bound for the persistent grid of the streaming kernels (0 = one
  * resident wave of workgroups on every CU) */
 void phist_amd_set_grid_blocks(int nblocks, int* iflag);
+/* workgroups per CU of the SpMV's persistent grid (0 = as many as fit; the
+ * default 2 keeps the x rows gathered by a wide band L2-resident) */
+void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
 
 /* the library's HIP stream (hipStream_t), device synchronisation, and an
  * event timer on that stream (ms between start and stop) */

@@ -85,6 +85,9 @@ struct Runtime {
   int sell_C = 0, sell_sigma = 0;
   // maximum persistent grid size for streaming kernels (0 = automatic)
   int grid_blocks = 0;
+  // workgroups per CU for the SpMV: the x gathers of a +-band thrash L2 when
+  // too many rows are in flight, two workgroups per CU measured best
+  int spmv_blocks_per_cu = 2;
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;

@@ -66,13 +66,15 @@ __device__ inline T wave_sum_strided(T v) {
   return v;
 }
 
-// Persistent-grid size for a kernel: resident workgroups, multiple of 8.
+// Persistent-grid size for a kernel: resident workgroups (at most
+// max_per_cu per CU when > 0), a multiple of 8.
 template <typename K>
-inline int persistent_blocks(K kernel, size_t lds_bytes, int64_t ntiles) {
+inline int persistent_blocks(K kernel, size_t lds_bytes, int64_t ntiles, int max_per_cu = 0) {
   Runtime& r = rt();
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds_bytes) != hipSuccess || per_cu < 1)
     per_cu = 1;
+  if (max_per_cu > 0) per_cu = std::min(per_cu, max_per_cu);
   int64_t g = (int64_t)r.num_cu * per_cu;
   if (r.grid_blocks > 0) g = std::min<int64_t>(g, r.grid_blocks);
   int64_t need = ((ntiles + kXcds - 1) / kXcds) * kXcds;

@@ -45,6 +45,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SELL_C")) r.sell_C = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SELL_SIGMA")) r.sell_sigma = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRID_BLOCKS")) r.grid_blocks = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
   kiss_reset();
   r.initialized = true;
 }
@@ -342,6 +343,14 @@ void phist_amd_set_grid_blocks(int nblocks, int* iflag) {
   guard(iflag, __func__, [&] {
     PHX_REQUIRE(nblocks >= 0, PHIST_INVALID_INPUT, "negative grid size");
     rt().grid_blocks = nblocks;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag) {
+  guard(iflag, __func__, [&] {
+    PHX_REQUIRE(n >= 0, PHIST_INVALID_INPUT, "negative workgroup count");
+    ensure_init();
+    rt().spmv_blocks_per_cu = n;
     *iflag = 0;
   });
 }

@@ -48,11 +48,50 @@ struct SpmvArgs {
 
 enum : int { F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16 };
 
+// Broadcast of lane `src` (0 <= src < LPR) of every LPR-lane group: DPP
+// quad permutes for groups of 2 and 4 lanes (VALU, no LDS traffic), a
+// cross-lane shuffle for wider groups.
+template <int LPR>
+__device__ inline int group_bcast(int v, int src, int lane) {
+  if constexpr (LPR == 1) {
+    return v;
+  } else if constexpr (LPR == 2) {
+    return src == 0 ? __builtin_amdgcn_mov_dpp(v, 0xA0, 0xF, 0xF, false)   // quad_perm [0,0,2,2]
+                    : __builtin_amdgcn_mov_dpp(v, 0xF5, 0xF, 0xF, false);  // quad_perm [1,1,3,3]
+  } else if constexpr (LPR == 4) {
+    switch (src) {
+      case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false);
+      case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false);
+      case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false);
+      default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false);
+    }
+  } else {
+    return __shfl(v, (lane & ~(LPR - 1)) + src, 64);
+  }
+}
+template <int LPR, typename T>
+__device__ inline T group_bcast_t(T v, int src, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(group_bcast<LPR>(__float_as_int(v), src, lane));
+  } else {
+    int2 p = *reinterpret_cast<int2*>(&v);
+    p.x = group_bcast<LPR>(p.x, src, lane);
+    p.y = group_bcast<LPR>(p.y, src, lane);
+    return *reinterpret_cast<T*>(&p);
+  }
+}
+
+// One row per LPR lanes; the lanes of a row load the row's val/col
+// cooperatively (lane q takes slot j = jb*LPR + q, coalesced over the rows
+// of the chunk) and share them by group broadcast, so every index is read
+// from memory once.  KB loads per lane are in flight per batch.
 template <typename T, int VPL, int LPR>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   constexpr int RPW = 64 / LPR;
   constexpr int RPB = (kBlock / 64) * RPW;
-  constexpr int UNR = 4;
+  constexpr int KB = LPR >= 16 ? 1 : (LPR >= 4 ? 2 : (LPR == 2 ? 4 : 8));
+  constexpr int BATCH = KB * LPR;  // slots per batch and row
+  constexpr int SUB = BATCH < 16 ? BATCH : 16;  // gathers in flight per lane
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int q = lane % LPR;
@@ -71,48 +110,66 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 
   const TileWalk tw = xcd_tiles(a.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    // all lanes stay in the loop body (group broadcasts need every lane)
     const int64_t s = t * RPB + wv * RPW + rw;
-    if (s >= a.nslots || !colok) continue;
-    const int32_t row = a.perm[s];
-    if (row < 0) continue;
+    const bool sok = s < a.nslots;
+    const int32_t row = sok ? a.perm[s] : -1;
+    const bool live = row >= 0;
     const int64_t ch = s >> a.clog2;
     const int r = (int)(s & (C - 1));
-    const int w = a.chunk_w[ch];
-    const int64_t base = a.chunk_ptr[ch] + r;
+    const int w = sok ? a.chunk_w[ch] : 0;
+    const int64_t base = sok ? a.chunk_ptr[ch] + r : 0;
+    // loop bound: the widest chunk among the wave's rows
+    int wmax = w;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
     const T* __restrict__ xb = a.x + c0;
 
     T acc[VPL], xo[VPL];
-    if (flags & (F_SHIFT | F_DVW)) vload<T, VPL>(xb + (int64_t)row * a.ldx, xo);
+#pragma unroll
+    for (int e = 0; e < VPL; e++) xo[e] = T(0);
+    if (live && colok && (flags & (F_SHIFT | F_DVW))) vload<T, VPL>(xb + (int64_t)row * a.ldx, xo);
 #pragma unroll
     for (int e = 0; e < VPL; e++) acc[e] = (flags & F_SHIFT) ? sh[e] * xo[e] : T(0);
 
-    int j = 0;
-    for (; j + UNR <= w; j += UNR) {
-      T v[UNR];
-      int32_t cj[UNR];
+    for (int j0 = 0; j0 < wmax; j0 += BATCH) {
+      T v[KB];
+      int32_t cj[KB];
 #pragma unroll
-      for (int k = 0; k < UNR; k++) {
-        const int64_t slot = base + (int64_t)(j + k) * C;
-        v[k] = __builtin_nontemporal_load(a.val + slot);
-        cj[k] = __builtin_nontemporal_load(a.col + slot);
+      for (int u = 0; u < KB; u++) {
+        const int j = j0 + u * LPR + q;
+        if (j < w) {
+          const int64_t slot = base + (int64_t)j * C;
+          v[u] = __builtin_nontemporal_load(a.val + slot);
+          cj[u] = __builtin_nontemporal_load(a.col + slot);
+        } else {
+          v[u] = T(0);
+          cj[u] = 0;
+        }
       }
-      T xg[UNR][VPL];
+      // slot j0+k of this row sits in lane k % LPR of the group, register k / LPR
 #pragma unroll
-      for (int k = 0; k < UNR; k++) vload<T, VPL>(xb + (int64_t)cj[k] * a.ldx, xg[k]);
+      for (int k0 = 0; k0 < BATCH; k0 += SUB) {
+        T vv[SUB];
+        T xg[SUB][VPL];
+        bool ok[SUB];
 #pragma unroll
-      for (int k = 0; k < UNR; k++)
+        for (int kk = 0; kk < SUB; kk++) {
+          const int k = k0 + kk;
+          const int32_t cc = group_bcast<LPR>(cj[k / LPR], k % LPR, lane);
+          vv[kk] = group_bcast_t<LPR>(v[k / LPR], k % LPR, lane);
+          ok[kk] = colok && (j0 + k) < w;
+          if (ok[kk]) vload<T, VPL>(xb + (int64_t)cc * a.ldx, xg[kk]);
+        }
 #pragma unroll
-        for (int e = 0; e < VPL; e++) acc[e] = fma(v[k], xg[k][e], acc[e]);
+        for (int kk = 0; kk < SUB; kk++)
+          if (ok[kk]) {
+#pragma unroll
+            for (int e = 0; e < VPL; e++) acc[e] = fma(vv[kk], xg[kk][e], acc[e]);
+          }
+      }
     }
-    for (; j < w; j++) {
-      const int64_t slot = base + (int64_t)j * C;
-      const T v = __builtin_nontemporal_load(a.val + slot);
-      const int32_t cj = __builtin_nontemporal_load(a.col + slot);
-      T xg[VPL];
-      vload<T, VPL>(xb + (int64_t)cj * a.ldx, xg);
-#pragma unroll
-      for (int e = 0; e < VPL; e++) acc[e] = fma(v, xg[e], acc[e]);
-    }
+    if (!live || !colok) continue;
 
     T yn[VPL];
     T* yp = a.y + (int64_t)row * a.ldy + c0;
@@ -177,7 +234,7 @@ int launch_one(SpmvArgs<T>& a) {
   auto kern = k_sell_spmv<T, VPL, LPR>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   a.ntiles = (a.nslots + RPB - 1) / RPB;
-  const int grid = persistent_blocks(kern, 0, a.ntiles);
+  const int grid = persistent_blocks(kern, 0, a.ntiles, rt().spmv_blocks_per_cu);
   if (a.flags & (F_DWW | F_DVW))
     a.partials = (T*)device_work(sizeof(T) * ((size_t)grid * 2 * a.ncols + 2 * a.ncols));
   ktimer_begin();
