@@ -109,6 +109,65 @@ def cpu_baseline(A, n_sample, nvec, budget_s=12.0):
             "ms_per_step": round(t * 1e3, 3)}
 
 
+def tall_skinny(be, args):
+    """Gram (self, V'V) or CholQR on N = 16,777,216 rows x k columns"""
+    L = be.L
+    iflag = ctypes.c_int(0)
+    N, k = 16777216, args.k
+    m = be.map(N)
+    v = be.mvec(m, k)
+    v.random()
+    C = be.sdmat(k, k)
+    if args.op == "gram":
+        f = be.fn("mvecT_times_mvec")
+        nbytes = N * k * 8
+
+        def step():
+            fl = ctypes.c_int(0)
+            f(ctypes.c_double(1.0), v.h, v.h, ctypes.c_double(0.0), C.h, ctypes.byref(fl))
+            if fl.value != 0:
+                raise RuntimeError(f"mvecT_times_mvec iflag={fl.value}")
+    else:
+        f = be.fn("chol_QR")
+        nbytes = 3 * N * k * 8
+        w0 = be.mvec(m, k)
+        be.call("mvec_add_mvec", be.s(1.0), v.h, be.s(0.0), w0.h)
+
+        def step():
+            be.call("mvec_add_mvec", be.s(1.0), w0.h, be.s(0.0), v.h)  # fresh input (not a timed kernel)
+            fl = ctypes.c_int(0)
+            f(v.h, C.h, ctypes.byref(fl))
+            if fl.value != 0:
+                raise RuntimeError(f"chol_QR iflag={fl.value}")
+    for _ in range(args.warmup):
+        step()
+    be.synchronize()
+    L.phist_amd_kernel_timer(1, ctypes.byref(iflag))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    be.synchronize()
+    t1 = time.perf_counter()
+    kms, kcount = ctypes.c_double(0.0), ctypes.c_int64(0)
+    L.phist_amd_kernel_timer_read(ctypes.byref(kms), ctypes.byref(kcount), ctypes.byref(iflag))
+    L.phist_amd_kernel_timer(0, ctypes.byref(iflag))
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    if args.op == "cholqr":
+        # the step also re-copies the input (2*N*k*8 bytes through k_axpby, not timed by the kernel timer);
+        # the kernel timer holds the Gram and the in-place update of each CholQR
+        k_ms = kms.value / args.steps
+    else:
+        k_ms = kms.value / max(1, kcount.value)
+    out = {"metric": f"{args.op} GB/s (N=16777216, k={k}, FP64)", "value": round(nbytes / (k_ms * 1e-3) / 1e9, 2),
+           "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic (KISS random block vector)",
+           "config": {"workload": f"{args.op} on 16,777,216 x {k} U(-1,1)", "bytes_per_op": nbytes,
+                      "kernel_ms_per_op": round(k_ms, 4), "kernel_launches_timed": kcount.value},
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / (k_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(nbytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +176,10 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--op", choices=["spmv", "gram", "cholqr"], default="spmv",
+                    help="spmv (default): the fused SpMV line; gram / cholqr: the tall-skinny configuration "
+                         "(16,777,216 x k, U(-1,1)) reported in the same format")
+    ap.add_argument("--k", type=int, default=8, help="block width for --op gram/cholqr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the library's STREAM copy kernel (1 GiB in, 1 GiB out per launch) so a "
                          "counter profile of this command holds a known-size reference dispatch")
@@ -131,6 +194,8 @@ def main():
     be.init()
     L = be.L
     iflag = ctypes.c_int(0)
+    if args.op != "spmv":
+        return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
     N, nvec = args.rows, args.nvec
     if args.copy_calibration:

@@ -88,6 +88,8 @@ struct Runtime {
   // workgroups per CU for the SpMV: the x gathers of a +-band thrash L2 when
   // too many rows are in flight, two workgroups per CU measured best
   int spmv_blocks_per_cu = 2;
+  // Gram products with both operands >= 16 columns run on the matrix cores
+  bool gram_mfma = true;
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;

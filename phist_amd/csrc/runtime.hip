@@ -46,6 +46,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SELL_SIGMA")) r.sell_sigma = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRID_BLOCKS")) r.grid_blocks = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   kiss_reset();
   r.initialized = true;
 }

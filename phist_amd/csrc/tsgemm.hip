@@ -87,6 +87,87 @@ __global__ void __launch_bounds__(kBlock) k_gram(const GramArgs<T> g) {
     }
 }
 
+// ---- MFMA Gram (both operands at least 16 columns) ----
+// v_mfma_{f64,f32}_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// (A 16x4, B 4x16) and holds four results of the 16x16 product; the result
+// row of register r is l/16 + 4r for f64 and 4*(l/16) + r for f32 (measured
+// on gfx950), the column is l%16.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// S = a'b for na, nb <= 32 (TM = ceil(na/16), TK = ceil(nb/16)); a wave
+// consumes 16 rows per iteration (four 4-row MFMA steps); with SELF (a == b)
+// only the upper block tiles are computed and mirrored.
+template <typename T, int TM, int TK, bool SELF>
+__global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
+  using M = Mfma<T>;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kk = lane >> 4, cc = lane & 15;
+  typename M::acc_t acc[TM][TK];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TK; j++) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
+  const TileWalk tw = xcd_tiles(g.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t row0 = t * 64 + wv * 16;
+    T a[4][TM], b[4][TK];
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      const int64_t r = row0 + st * 4 + kk;
+      const bool ok = r < g.nrows;
+#pragma unroll
+      for (int tm = 0; tm < TM; tm++) {
+        const int col = tm * 16 + cc;
+        a[st][tm] = (ok && col < g.na) ? g.a[r * g.lda + col] : T(0);
+      }
+      if constexpr (!SELF) {
+#pragma unroll
+        for (int tk = 0; tk < TK; tk++) {
+          const int col = tk * 16 + cc;
+          b[st][tk] = (ok && col < g.nb) ? g.b[r * g.ldb + col] : T(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < 4; st++)
+#pragma unroll
+      for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+        for (int tk = 0; tk < TK; tk++) {
+          if (SELF && tm > tk) continue;
+          acc[tm][tk] = M::op(a[st][tm], SELF ? a[st][tk] : b[st][tk], acc[tm][tk]);
+        }
+  }
+  const int nout = g.na * g.nb;
+  T* part = g.partials + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * nout;
+#pragma unroll
+  for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+    for (int tk = 0; tk < TK; tk++) {
+      if (SELF && tm > tk) continue;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = tm * 16 + M::row(lane, r), j = tk * 16 + cc;
+        const T v = acc[tm][tk][r];
+        if (i < g.na && j < g.nb) part[j * g.na + i] = v;
+        if (SELF && tm < tk && j < g.na && i < g.nb) part[i * g.na + j] = v;
+      }
+    }
+}
+
 // C(r,c) = alpha * S + beta * C with S summed over the block partials;
 // S is stored c*na + r, transposed when the operands were swapped
 template <typename T>
@@ -136,6 +217,53 @@ int gram_lpr(GramArgs<T>& g, int lpr) {
 
 }  // namespace
 
+template <typename T, int TM, int TK, bool SELF>
+static int gram_mfma_launch(GramArgs<T>& g) {
+  auto kern = k_gram_mfma<T, TM, TK, SELF>;
+  g.ntiles = (g.nrows + 63) / 64;
+  const int grid = persistent_blocks(kern, 0, g.ntiles);
+  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * g.na * g.nb);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, g);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+  return grid * (kBlock / 64);
+}
+
+// S = a'b on the matrix cores for na, nb <= 32 (zero-padded to 16-column
+// tiles), written to C as C = alpha*S + beta*C; Cmirror, when given,
+// receives the transposed block from the same partial sums
+template <typename T>
+static void gram_block_mfma(T alpha, const T* a, int64_t lda, int na, const T* b, int64_t ldb, int nb, int64_t nrows,
+                            bool self, T beta, T* Cdev, int ldc, T* Cmirror = nullptr) {
+  GramArgs<T> g;
+  g.nrows = nrows;
+  g.a = a;
+  g.lda = lda;
+  g.na = na;
+  g.b = b;
+  g.ldb = ldb;
+  g.nb = nb;
+  const int tm = (na + 15) / 16, tk = (nb + 15) / 16;
+  int nparts;
+  if (self && tm == 1) nparts = gram_mfma_launch<T, 1, 1, true>(g);
+  else if (self) nparts = gram_mfma_launch<T, 2, 2, true>(g);
+  else if (tm == 1 && tk == 1) nparts = gram_mfma_launch<T, 1, 1, false>(g);
+  else if (tm == 1) nparts = gram_mfma_launch<T, 1, 2, false>(g);
+  else if (tk == 1) nparts = gram_mfma_launch<T, 2, 1, false>(g);
+  else nparts = gram_mfma_launch<T, 2, 2, false>(g);
+  const int nout = na * nb;
+  const int fblocks = std::max(1, std::min(64, (nout + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, na, nb,
+                     false, alpha, beta, Cdev, ldc);
+  PHX_HIP(hipGetLastError());
+  if (Cmirror) {  // the transposed block of a self Gram, from the same partials
+    hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, na, nb,
+                       true, alpha, beta, Cmirror, ldc);
+    PHX_HIP(hipGetLastError());
+  }
+}
+
 // Gram of the column blocks a (na cols) and b (nb <= 32 cols, nb <= na):
 // S = a'b, written to C as C = alpha*S + beta*C (or its transpose); the a
 // side is cut into pieces one lane group covers (32 lanes of VPL columns)
@@ -184,6 +312,24 @@ void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C)
   const int ldc = C->root->ld;
   // the sliced operand is the wider one; the broadcast side is cut in blocks
   // of at most 32 columns (each block one pass over the rows)
+  if (m >= 16 && k >= 16 && rt().gram_mfma) {
+    // matrix cores: 32 x 32 output blocks (the self Gram computes the upper
+    // blocks and mirrors them)
+    const bool self = V->root == W->root && V->jofs == W->jofs && m == k;
+    for (int i0 = 0; i0 < m; i0 += 32)
+      for (int j0 = 0; j0 < k; j0 += 32) {
+        const int mi = std::min(32, m - i0), kj = std::min(32, k - j0);
+        T* cblk = C->dev() + (int64_t)j0 * ldc + i0;
+        if (self && i0 > j0) continue;
+        const bool diag_self = self && i0 == j0;
+        // off-diagonal blocks of a self Gram also fill their mirror C(j0.., i0..)
+        T* cmir = (self && i0 < j0) ? C->dev() + (int64_t)i0 * ldc + j0 : nullptr;
+        gram_block_mfma<T>(alpha, V->dev() + i0, V->lda, mi, W->dev() + j0, W->lda, kj, V->nrows, diag_self, beta,
+                           cblk, ldc, cmir);
+      }
+    sd_from_device(C);
+    return;
+  }
   const bool swap = k > m;
   const Mvec<T>* A = swap ? W : V;
   const Mvec<T>* B = swap ? V : W;

@@ -120,7 +120,7 @@ def test_random_is_kiss_bitexact(be64):
 
 
 @pytest.mark.parametrize("m,k", [(1, 1), (4, 4), (8, 8), (3, 5), (16, 16), (32, 32), (32, 4), (2, 40), (64, 8),
-                                 (7, 7), (130, 3)])
+                                 (7, 7), (130, 3), (16, 24), (17, 33), (40, 40), (64, 48), (20, 16)])
 @pytest.mark.parametrize("beta", [0.0, -0.5])
 def test_gram(be, m, k, beta):
     n = 5003
@@ -136,6 +136,25 @@ def test_gram(be, m, k, beta):
     bound = 8 * RTOL[be.dtype.type] * (0.8 * np.abs(Vt).T @ np.abs(Wt) + abs(beta) * np.abs(Ct))
     assert np.all(np.abs(C.get().astype(float) - want) <= bound)
     for o in (v, w, C):
+        o.delete()
+
+
+@pytest.mark.parametrize("k", [16, 24, 32, 40, 64])
+def test_gram_self_mfma(be, k):
+    """V'V on the matrix cores (upper tiles computed, mirrored) vs the oracle"""
+    n = 4099
+    V = np.random.default_rng(k).uniform(-1, 1, (n, k))
+    mp = be.map(n)
+    v = be.mvec(mp, k, V)
+    C0 = np.random.default_rng(1).uniform(-1, 1, (k, k))
+    C = be.sdmat(k, k, C0)
+    be.gram(1.5, v, v, 0.5, C)
+    Vt, Ct = V.astype(be.dtype).astype(float), C0.astype(be.dtype).astype(float)
+    want = ol.gram(1.5, Vt, Vt, 0.5, Ct)
+    bound = 8 * RTOL[be.dtype.type] * (1.5 * np.abs(Vt).T @ np.abs(Vt) + 0.5 * np.abs(Ct))
+    got = C.get().astype(float)
+    assert np.all(np.abs(got - want) <= bound)
+    for o in (v, C):
         o.delete()
 
 
