@@ -90,6 +90,11 @@ struct Runtime {
   int spmv_blocks_per_cu = 2;
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
+  // workgroups per CU of the Gram kernels (each wave leaves one partial tile,
+  // so this also bounds the finalize work)
+  int gram_blocks_per_cu = 8;
+  // row-mixing products (V*C, in-place V*M) with k, m >= 8 run on the matrix cores
+  bool mix_mfma = true;
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;

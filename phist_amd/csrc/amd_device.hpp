@@ -5,8 +5,8 @@
 //   workgroups of an XCD walk their range together; the set of rows in flight
 //   on one XCD (and the x rows they gather) then stays narrow and L2-resident.
 // * 16-byte vector loads/stores of VPL consecutive scalars.
-// * deterministic two-stage reductions: each workgroup writes its partial
-//   sums, a single workgroup adds them in a fixed order.
+// * deterministic two-stage reductions: each workgroup (or wave) writes its
+//   partial sums, a second kernel adds them in a fixed order.
 #pragma once
 #include "amd_common.hpp"
 
@@ -83,23 +83,36 @@ inline int persistent_blocks(K kernel, size_t lds_bytes, int64_t ntiles, int max
   return (int)g;
 }
 
-// Adds partials[b*width + i] over b in a fixed order; out[i] (device).
+// Sum over b of partials[b*width + o] for the 16 outputs o = 16*blockIdx.x +
+// (tid & 15) of this workgroup: thread group p = tid / 16 adds b = p, p+16, ...
+// in order, then the 16 group sums are added in order of p.  The result is
+// returned on threads 0..15 (others get 0); every thread must call it.
+template <typename T>
+__device__ inline T block_sum16(const T* __restrict__ partials, int nblocks, int width) {
+  __shared__ T sh[16][17];
+  const int o = blockIdx.x * 16 + (threadIdx.x & 15), grp = threadIdx.x >> 4;
+  T s = T(0);
+  if (o < width) {
+#pragma unroll 8
+    for (int b = grp; b < nblocks; b += kBlock / 16) s += partials[(int64_t)b * width + o];
+  }
+  sh[grp][threadIdx.x & 15] = s;
+  __syncthreads();
+  T t = T(0);
+  if (threadIdx.x < 16)
+#pragma unroll
+    for (int q = 0; q < kBlock / 16; q++) t += sh[q][threadIdx.x];
+  return t;
+}
+
+// Adds partials[b*width + i] over b in a fixed order; out[i] (device);
+// launched with ceil(width / 16) workgroups.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_reduce_partials(const T* __restrict__ partials, int nblocks, int width,
                                                             T* __restrict__ out) {
-  __shared__ T sh[kBlock];
-  for (int i = 0; i < width; i++) {
-    T s = T(0);
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partials[(int64_t)b * width + i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-      if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[i] = sh[0];
-    __syncthreads();
-  }
+  const T s = block_sum16(partials, nblocks, width);
+  const int o = blockIdx.x * 16 + threadIdx.x;
+  if (threadIdx.x < 16 && o < width) out[o] = s;
 }
 
 // reduce [nblocks][width] device partials and copy the width sums to host_out
@@ -107,7 +120,8 @@ template <typename T>
 inline void reduce_partials_to_host(const T* d_partials, int nblocks, int width, T* host_out) {
   Runtime& r = rt();
   T* d_out = const_cast<T*>(d_partials) + (size_t)nblocks * width;
-  hipLaunchKernelGGL(k_reduce_partials<T>, dim3(1), dim3(kBlock), 0, r.stream, d_partials, nblocks, width, d_out);
+  hipLaunchKernelGGL(k_reduce_partials<T>, dim3((width + 15) / 16), dim3(kBlock), 0, r.stream, d_partials, nblocks,
+                     width, d_out);
   PHX_HIP(hipGetLastError());
   T* h = (T*)host_work(sizeof(T) * width);
   PHX_HIP(hipMemcpyAsync(h, d_out, sizeof(T) * width, hipMemcpyDeviceToHost, r.stream));

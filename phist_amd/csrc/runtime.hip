@@ -47,6 +47,8 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_GRID_BLOCKS")) r.grid_blocks = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_GRAM_BLOCKS_PER_CU")) r.gram_blocks_per_cu = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_MIX_MFMA")) r.mix_mfma = atoi(s) != 0;
   kiss_reset();
   r.initialized = true;
 }

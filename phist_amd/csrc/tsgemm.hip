@@ -168,16 +168,16 @@ __global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
     }
 }
 
-// C(r,c) = alpha * S + beta * C with S summed over the block partials;
-// S is stored c*na + r, transposed when the operands were swapped
+// C(r,c) = alpha * S + beta * C with S summed over the block partials in a
+// fixed order; S is stored c*na + r, transposed when the operands were swapped
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_gram_finalize(const T* __restrict__ partials, int nblocks, int na, int nb,
                                                           bool transposed, T alpha, T beta, T* __restrict__ C,
                                                           int ldc) {
   const int nout = na * nb;
-  for (int o = blockIdx.x * kBlock + threadIdx.x; o < nout; o += gridDim.x * kBlock) {
-    T s = T(0);
-    for (int b = 0; b < nblocks; b++) s += partials[(int64_t)b * nout + o];
+  const T s = block_sum16(partials, nblocks, nout);
+  const int o = blockIdx.x * 16 + threadIdx.x;
+  if (threadIdx.x < 16 && o < nout) {
     const int r = o % na, c = o / na;
     const int ci = transposed ? r : c, ri = transposed ? c : r;
     T* cp = C + (int64_t)ci * ldc + ri;
@@ -194,7 +194,7 @@ int gram_launch_one(GramArgs<T>& g) {
   auto kern = k_gram<T, VPL, LPR, KB>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   g.ntiles = (g.nrows + RPB - 1) / RPB;
-  const int grid = persistent_blocks(kern, 0, g.ntiles);
+  const int grid = persistent_blocks(kern, 0, g.ntiles, rt().gram_blocks_per_cu);
   g.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * g.na * g.nb);
   ktimer_begin();
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, g);
@@ -221,7 +221,7 @@ template <typename T, int TM, int TK, bool SELF>
 static int gram_mfma_launch(GramArgs<T>& g) {
   auto kern = k_gram_mfma<T, TM, TK, SELF>;
   g.ntiles = (g.nrows + 63) / 64;
-  const int grid = persistent_blocks(kern, 0, g.ntiles);
+  const int grid = persistent_blocks(kern, 0, g.ntiles, rt().gram_blocks_per_cu);
   g.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * g.na * g.nb);
   ktimer_begin();
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, g);
@@ -253,7 +253,7 @@ static void gram_block_mfma(T alpha, const T* a, int64_t lda, int na, const T* b
   else if (tk == 1) nparts = gram_mfma_launch<T, 2, 1, false>(g);
   else nparts = gram_mfma_launch<T, 2, 2, false>(g);
   const int nout = na * nb;
-  const int fblocks = std::max(1, std::min(64, (nout + kBlock - 1) / kBlock));
+  const int fblocks = (nout + 15) / 16;
   hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, na, nb,
                      false, alpha, beta, Cdev, ldc);
   PHX_HIP(hipGetLastError());
@@ -296,7 +296,7 @@ static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int
     // S rows p0.. of this piece land in C rows (or columns, transposed) p0..
     T* cpiece = transposed ? Cdev + (int64_t)p0 * ldc : Cdev + p0;
     const int nout = g.na * nb;
-    const int fblocks = std::max(1, std::min(64, (nout + kBlock - 1) / kBlock));
+    const int fblocks = (nout + 15) / 16;
     hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, g.na, nb,
                        transposed, alpha, beta, cpiece, ldc);
     PHX_HIP(hipGetLastError());
@@ -436,6 +436,82 @@ __global__ void __launch_bounds__(kBlock) k_rowmix(const MixArgs<T> p) {
   }
 }
 
+// Row-mixing on the matrix cores (modes 0 and 2): a wave forms RT tiles of
+// 16 rows x 16*TM columns per iteration as sum_s A_s * B_s over KS 16x4x16
+// steps.  The k columns of V are dealt to the four lane groups g = lane/16 as
+// column g*KS + s of step s, so a lane reads KS consecutive entries of its
+// row and the wave reads each row exactly once; B_s = alpha*C(g*KS + s,
+// 16*tm + lane%16) stays in registers for the whole launch.  All loads of a
+// tile precede its stores (the stores depend on every MFMA of the tile) and
+// each row belongs to one wave, so W may alias V (mode 2).
+template <typename T, int KS, int TM, int RT>
+__global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
+  using M = Mfma<T>;
+  constexpr int V16 = 16 / sizeof(T);
+  constexpr int ROWS = 16 * RT;  // rows per wave and iteration
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, cc = lane & 15;
+  const int j0 = g * KS;
+  T bC[KS][TM];
+#pragma unroll
+  for (int st = 0; st < KS; st++)
+#pragma unroll
+    for (int tm = 0; tm < TM; tm++) {
+      const int j = j0 + st, c = tm * 16 + cc;
+      bC[st][tm] = (j < p.k && c < p.m) ? p.alpha * p.cdev[(int64_t)c * p.ldc + j] : T(0);
+    }
+  const bool vrow = p.vvec && (KS % V16 == 0) && j0 + KS <= p.k;
+  const TileWalk tw = xcd_tiles(p.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t r0 = t * (4 * ROWS) + wv * ROWS;
+    T a[RT][KS];
+#pragma unroll
+    for (int rt_ = 0; rt_ < RT; rt_++) {
+      const int64_t ra = r0 + rt_ * 16 + cc;
+      const T* vp = p.v + ra * p.ldv + j0;
+      if (ra < p.nrows && vrow) {
+#pragma unroll
+        for (int u = 0; u < KS; u += (KS % V16 == 0 ? V16 : 1)) {
+          if constexpr (KS % V16 == 0) {
+            T tmp[V16];
+            vload<T, V16>(vp + u, tmp);
+#pragma unroll
+            for (int e = 0; e < V16; e++) a[rt_][u + e] = tmp[e];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int st = 0; st < KS; st++) a[rt_][st] = (ra < p.nrows && j0 + st < p.k) ? vp[st] : T(0);
+      }
+    }
+    typename M::acc_t acc[RT][TM];
+#pragma unroll
+    for (int rt_ = 0; rt_ < RT; rt_++)
+#pragma unroll
+      for (int tm = 0; tm < TM; tm++) {
+        acc[rt_][tm] = typename M::acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < KS; st++) acc[rt_][tm] = M::op(a[rt_][st], bC[st][tm], acc[rt_][tm]);
+      }
+#pragma unroll
+    for (int rt_ = 0; rt_ < RT; rt_++)
+#pragma unroll
+      for (int tm = 0; tm < TM; tm++) {
+        const int c = tm * 16 + cc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int64_t row = r0 + rt_ * 16 + M::row(lane, r);
+          if (row < p.nrows && c < p.m) {
+            T* wp = p.w + row * p.ldw + c;
+            T val = acc[rt_][tm][r];
+            if (p.mode == 0 && p.beta != T(0)) val = fma(p.beta, *wp, val);
+            *wp = val;
+          }
+        }
+      }
+  }
+}
+
 namespace {
 
 template <typename T, int VPL, int LPR>
@@ -466,6 +542,43 @@ void mix_lpr(MixArgs<T>& p, int lpr, size_t lds) {
 constexpr size_t kMixLds = 32 * 1024;
 
 // one launch; the caller guarantees the LDS footprint fits and m <= 64*VPL
+template <typename T, int KS, int TM>
+void mix_mfma_launch(MixArgs<T>& p) {
+  constexpr int RT = KS >= 8 ? 1 : 2;
+  auto kern = k_rowmix_mfma<T, KS, TM, RT>;
+  p.ntiles = (p.nrows + 64 * RT - 1) / (64 * RT);
+  const int grid = persistent_blocks(kern, 0, p.ntiles);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, p);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+}
+
+template <typename T, int KS>
+void mix_mfma_tm(MixArgs<T>& p) {
+  switch ((p.m + 15) / 16) {
+    case 1: return mix_mfma_launch<T, KS, 1>(p);
+    case 2: return mix_mfma_launch<T, KS, 2>(p);
+    case 3: return mix_mfma_launch<T, KS, 3>(p);
+    default: return mix_mfma_launch<T, KS, 4>(p);
+  }
+}
+
+// modes 0 and 2 with 8 <= k, m <= 64 go to the matrix cores
+template <typename T>
+bool mix_mfma_ok(int mode, int k, int m) {
+  return rt().mix_mfma && mode != 1 && k >= 8 && m >= 8 && k <= 64 && m <= 64;
+}
+
+template <typename T>
+void mix_mfma(MixArgs<T>& p) {
+  const int ks = (p.k + 3) / 4;
+  if (ks <= 2) mix_mfma_tm<T, 2>(p);
+  else if (ks <= 4) mix_mfma_tm<T, 4>(p);
+  else if (ks <= 8) mix_mfma_tm<T, 8>(p);
+  else mix_mfma_tm<T, 16>(p);
+}
+
 template <typename T>
 void rowmix(int mode, T alpha, const T* v, int64_t ldv, int k, const T* cdev, int ldc, T beta, T* w, int64_t ldw,
             int m, const T* ddev, int ldd, int64_t nrows) {
@@ -489,6 +602,7 @@ void rowmix(int mode, T alpha, const T* v, int64_t ldv, int k, const T* cdev, in
   p.beta = beta;
   p.mode = mode;
   p.vvec = vvec ? 1 : 0;
+  if (mix_mfma_ok<T>(mode, k, m)) return mix_mfma<T>(p);
   const size_t lds = sizeof(T) * ((size_t)k * m + (mode == 1 ? (size_t)m * m : 0));
   PHX_REQUIRE(m <= 64 * VPL && lds <= kMixLds, PHIST_FUNCTIONAL_ERROR, "row-mix launch of %d x %d exceeds one pass", k,
               m);
@@ -524,6 +638,7 @@ void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>
   for (int c0 = 0; c0 < m; c0 += mb) {
     const int mc = std::min(mb, m - c0);
     int kb = std::max(1, (int)(kMixLds / sizeof(T)) / std::max(1, mc));
+    if (rt().mix_mfma && mc >= 8 && k > 64) kb = std::min(kb, 64);  // k blocks the matrix-core kernel takes
     for (int j0 = 0; j0 < k || (k == 0 && j0 == 0); j0 += kb) {
       const int kc = std::min(kb, k - j0);
       const T b = j0 == 0 ? beta : T(1);

@@ -171,7 +171,8 @@ def test_gram_self_and_views(be64):
         o.delete()
 
 
-@pytest.mark.parametrize("k,m", [(1, 1), (4, 4), (8, 8), (5, 3), (16, 16), (32, 32), (40, 6), (3, 33)])
+@pytest.mark.parametrize("k,m", [(1, 1), (4, 4), (8, 8), (5, 3), (16, 16), (32, 32), (40, 6), (3, 33), (8, 33),
+                                 (12, 20), (64, 64), (64, 17), (100, 40), (9, 70)])
 @pytest.mark.parametrize("beta", [0.0, 0.5])
 def test_times_sdmat(be, k, m, beta):
     n = 3001
@@ -188,7 +189,8 @@ def test_times_sdmat(be, k, m, beta):
         o.delete()
 
 
-@pytest.mark.parametrize("k,m", [(1, 1), (4, 4), (8, 8), (6, 3), (16, 16), (32, 32), (33, 17)])
+@pytest.mark.parametrize("k,m", [(1, 1), (4, 4), (8, 8), (6, 3), (16, 16), (32, 32), (33, 17), (64, 64), (24, 9),
+                                 (40, 40), (70, 20)])
 def test_times_sdmat_inplace(be, k, m):
     n = 2049
     rng = np.random.default_rng(k + 3 * m)
@@ -202,6 +204,31 @@ def test_times_sdmat_inplace(be, k, m):
     assert rel(got[:, :m], ol.times_sdmat_inplace(Vt, Mt)) <= 4 * RTOL[be.dtype.type] * max(1, k) ** 0.5
     assert np.array_equal(got[:, m:], Vt[:, m:])  # columns past m untouched
     for o in (v, M):
+        o.delete()
+
+
+@pytest.mark.parametrize("j0,k,m", [(1, 17, 9), (2, 16, 16), (3, 12, 12), (0, 8, 8)])
+def test_times_sdmat_views(be, j0, k, m):
+    """row-mixing on column views (unaligned rows take the scalar-load path)"""
+    n, ncol = 1537, 20
+    rng = np.random.default_rng(j0 + k)
+    X, C0, M0 = rng.uniform(-1, 1, (n, ncol)), rng.uniform(-1, 1, (k, m)), rng.uniform(-1, 1, (k, m))
+    mp = be.map(n)
+    x = be.mvec(mp, ncol, X)
+    w = be.mvec(mp, m)
+    xv = x.view(j0, j0 + k - 1)
+    C, M = be.sdmat(k, m, C0), be.sdmat(k, m, M0)
+    Xt, Ct, Mt = (a.astype(be.dtype).astype(float) for a in (X, C0, M0))
+    tol = 4 * RTOL[be.dtype.type] * k ** 0.5
+    be.times_sdmat(1.0, xv, C, 0.0, w)
+    assert rel(w.get().astype(float), Xt[:, j0:j0 + k] @ Ct) <= tol
+    be.times_sdmat_inplace(xv, M)
+    got = x.get().astype(float)
+    want = Xt.copy()
+    want[:, j0:j0 + m] = Xt[:, j0:j0 + k] @ Mt
+    assert rel(got, want) <= tol
+    assert np.array_equal(got[:, :j0], Xt[:, :j0]) and np.array_equal(got[:, j0 + m:], Xt[:, j0 + m:])
+    for o in (xv, x, w, C, M):
         o.delete()
 
 
