@@ -110,7 +110,7 @@ def cpu_baseline(A, n_sample, nvec, budget_s=12.0):
 
 
 def tall_skinny(be, args):
-    """Gram (self, V'V) or CholQR on N = 16,777,216 rows x k columns"""
+    """Gram (self, V'V), CholQR or fused_mvsd_mvTmv on N = 16,777,216 rows x k columns"""
     L = be.L
     iflag = ctypes.c_int(0)
     N, k = 16777216, args.k
@@ -127,6 +127,17 @@ def tall_skinny(be, args):
             f(ctypes.c_double(1.0), v.h, v.h, ctypes.c_double(0.0), C.h, ctypes.byref(fl))
             if fl.value != 0:
                 raise RuntimeError(f"mvecT_times_mvec iflag={fl.value}")
+    elif args.op == "mvsd":
+        f = be.fn("fused_mvsd_mvTmv")
+        nbytes = 2 * N * k * 8
+        w = be.mvec(m, k)
+        Cm = be.sdmat(k, k, np.random.default_rng(3).uniform(-1, 1, (k, k)))
+
+        def step():
+            fl = ctypes.c_int(0)
+            f(ctypes.c_double(1.0), v.h, Cm.h, ctypes.c_double(0.0), w.h, C.h, ctypes.byref(fl))
+            if fl.value != 0:
+                raise RuntimeError(f"fused_mvsd_mvTmv iflag={fl.value}")
     else:
         f = be.fn("chol_QR")
         nbytes = 3 * N * k * 8
@@ -176,9 +187,10 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--op", choices=["spmv", "gram", "cholqr"], default="spmv",
-                    help="spmv (default): the fused SpMV line; gram / cholqr: the tall-skinny configuration "
-                         "(16,777,216 x k, U(-1,1)) reported in the same format")
+    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd"], default="spmv",
+                    help="spmv (default): the fused SpMV line; gram / cholqr / mvsd (fused_mvsd_mvTmv, "
+                         "W = V*C and W'W, beta=0): the tall-skinny configuration (16,777,216 x k, U(-1,1)) "
+                         "reported in the same format")
     ap.add_argument("--k", type=int, default=8, help="block width for --op gram/cholqr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the library's STREAM copy kernel (1 GiB in, 1 GiB out per launch) so a "

@@ -362,6 +362,12 @@ struct MixArgs {
   T alpha, beta;
   int mode;  // 0: W = alpha*V*C + beta*W, 1: W = W*D + V*C, 2: W = V*C (W may alias V)
   int vvec;  // V and W rows may be read with 16-byte vectors
+  // Gram epilogue of the matrix-core kernel: S = G'Wnew (G = gv, ngv columns)
+  // or Wnew'Wnew, one partial tile per wave in partials
+  const T* gv;
+  int64_t ldgv;
+  int ngv;
+  T* partials;
 };
 
 template <typename T, int VPL, int LPR>
@@ -444,9 +450,23 @@ __global__ void __launch_bounds__(kBlock) k_rowmix(const MixArgs<T> p) {
 // 16*tm + lane%16) stays in registers for the whole launch.  All loads of a
 // tile precede its stores (the stores depend on every MFMA of the tile) and
 // each row belongs to one wave, so W may alias V (mode 2).
-template <typename T, int KS, int TM, int RT>
+//
+// GRAM = 1 adds Wnew'Wnew (upper 16x16 tiles, mirrored), GRAM = 2 adds
+// G'Wnew for a second block vector G of at most 16*TV columns: the new W
+// values sit in the MFMA result layout, which is the operand layout of a
+// Gram step over the same four rows, so the product needs no extra traffic
+// for W.
+template <typename T, int KS, int TM, int RT, int GRAM = 0, int TV = 1>
 __global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
   using M = Mfma<T>;
+  constexpr int TG = GRAM == 1 ? TM : TV;  // tiles of the Gram's left operand
+  typename M::acc_t gacc[TG][TM];
+  if constexpr (GRAM != 0) {
+#pragma unroll
+    for (int i = 0; i < TG; i++)
+#pragma unroll
+      for (int j = 0; j < TM; j++) gacc[i][j] = typename M::acc_t{0, 0, 0, 0};
+  }
   constexpr int V16 = 16 / sizeof(T);
   constexpr int ROWS = 16 * RT;  // rows per wave and iteration
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -494,7 +514,7 @@ __global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
         for (int st = 0; st < KS; st++) acc[rt_][tm] = M::op(a[rt_][st], bC[st][tm], acc[rt_][tm]);
       }
 #pragma unroll
-    for (int rt_ = 0; rt_ < RT; rt_++)
+    for (int rt_ = 0; rt_ < RT; rt_++) {
 #pragma unroll
       for (int tm = 0; tm < TM; tm++) {
         const int c = tm * 16 + cc;
@@ -506,7 +526,50 @@ __global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
             T val = acc[rt_][tm][r];
             if (p.mode == 0 && p.beta != T(0)) val = fma(p.beta, *wp, val);
             *wp = val;
+            acc[rt_][tm][r] = val;  // the Gram epilogue uses the stored values
           }
+        }
+      }
+      if constexpr (GRAM == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int tg = 0; tg < TM; tg++)
+#pragma unroll
+            for (int tm = tg; tm < TM; tm++) gacc[tg][tm] = M::op(acc[rt_][tg][r], acc[rt_][tm][r], gacc[tg][tm]);
+      } else if constexpr (GRAM == 2) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int64_t row = r0 + rt_ * 16 + M::row(lane, r);
+          T g[TV];
+#pragma unroll
+          for (int tv = 0; tv < TV; tv++) {
+            const int c = tv * 16 + cc;
+            g[tv] = (row < p.nrows && c < p.ngv) ? p.gv[row * p.ldgv + c] : T(0);
+          }
+#pragma unroll
+          for (int tv = 0; tv < TV; tv++)
+#pragma unroll
+            for (int tm = 0; tm < TM; tm++) gacc[tv][tm] = M::op(g[tv], acc[rt_][tm][r], gacc[tv][tm]);
+        }
+      }
+    }
+  }
+  if constexpr (GRAM != 0) {
+    // partial tile of this wave: S(i, j) at j*na + i
+    const int na = GRAM == 1 ? p.m : p.ngv, nb = p.m;
+    T* part = p.partials + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * na * nb;
+#pragma unroll
+    for (int tg = 0; tg < TG; tg++)
+#pragma unroll
+      for (int tm = 0; tm < TM; tm++) {
+        if (GRAM == 1 && tg > tm) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = tg * 16 + M::row(lane, r), j = tm * 16 + cc;
+          const T v = gacc[tg][tm][r];
+          if (i < na && j < nb) part[j * na + i] = v;
+          if (GRAM == 1 && tg < tm && j < na && i < nb) part[i * na + j] = v;
         }
       }
   }
@@ -562,6 +625,34 @@ void mix_mfma_tm(MixArgs<T>& p) {
     case 3: return mix_mfma_launch<T, KS, 3>(p);
     default: return mix_mfma_launch<T, KS, 4>(p);
   }
+}
+
+template <typename T, int KS, int TM, int GRAM, int TV>
+int mix_gram_launch(MixArgs<T>& p) {
+  constexpr int RT = KS >= 8 ? 1 : 2;
+  auto kern = k_rowmix_mfma<T, KS, TM, RT, GRAM, TV>;
+  p.ntiles = (p.nrows + 64 * RT - 1) / (64 * RT);
+  const int grid = persistent_blocks(kern, 0, p.ntiles, rt().gram_blocks_per_cu);
+  const int na = GRAM == 1 ? p.m : p.ngv;
+  p.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * na * p.m);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, p);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+  return grid * (kBlock / 64);
+}
+
+template <typename T, int KS, int TM>
+int mix_gram_tv(MixArgs<T>& p, bool self) {
+  if (self) return mix_gram_launch<T, KS, TM, 1, 1>(p);
+  if (p.ngv <= 16) return mix_gram_launch<T, KS, TM, 2, 1>(p);
+  return mix_gram_launch<T, KS, TM, 2, 2>(p);
+}
+
+template <typename T, int KS>
+int mix_gram_tm(MixArgs<T>& p, bool self) {
+  if (p.m <= 16) return mix_gram_tv<T, KS, 1>(p, self);
+  return mix_gram_tv<T, KS, 2>(p, self);
 }
 
 // modes 0 and 2 with 8 <= k, m <= 64 go to the matrix cores
@@ -623,6 +714,49 @@ bool mix_fits(int k, int m, bool with_d) {
 }
 
 }  // namespace
+
+// One sweep for W = alpha*V*C + beta*W (mode 0) or W(:,1:m) = W*C in place
+// (mode 2, V == W) together with D = ga*S + gb*D, S = Wnew'Wnew (G == nullptr)
+// or G'Wnew.  Returns false when the shapes are outside the fused kernel
+// (8 <= k <= 64, 8 <= m <= 32, G at most 32 columns, G not overlapping W).
+template <typename T>
+bool mix_gram(int mode, T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W, const Mvec<T>* G, T ga,
+              T gb, SdMat<T>* D) {
+  const int k = V->nvec, m = C->ncols;
+  if (!rt().mix_mfma || k < 8 || k > 64 || m < 8 || m > 32) return false;
+  if (G && (G->nvec > 32 || G->nrows != W->nrows)) return false;
+  if (G && G->root == W->root && !(G->jofs + G->nvec <= W->jofs || W->jofs + m <= G->jofs)) return false;
+  constexpr int V16 = 16 / sizeof(T);
+  MixArgs<T> p{};
+  p.nrows = V->nrows;
+  p.v = V->dev();
+  p.ldv = V->lda;
+  p.k = k;
+  p.w = W->dev();
+  p.ldw = W->lda;
+  p.m = m;
+  p.cdev = C->dev();
+  p.ldc = C->root->ld;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.mode = mode;
+  p.vvec = (p.ldv % V16 == 0) && al16(p.v) ? 1 : 0;
+  p.gv = G ? G->dev() : nullptr;
+  p.ldgv = G ? G->lda : 0;
+  p.ngv = G ? G->nvec : 0;
+  const int ks = (k + 3) / 4;
+  int nparts;
+  if (ks <= 2) nparts = mix_gram_tm<T, 2>(p, !G);
+  else if (ks <= 4) nparts = mix_gram_tm<T, 4>(p, !G);
+  else if (ks <= 8) nparts = mix_gram_tm<T, 8>(p, !G);
+  else nparts = mix_gram_tm<T, 16>(p, !G);
+  const int na = G ? G->nvec : m, nout = na * m;
+  hipLaunchKernelGGL(k_gram_finalize<T>, dim3((nout + 15) / 16), dim3(kBlock), 0, rt().stream, p.partials, nparts, na,
+                     m, false, ga, gb, D->dev(), D->root->ld);
+  PHX_HIP(hipGetLastError());
+  sd_from_device(D);
+  return true;
+}
 
 template <typename T>
 void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W) {
@@ -686,6 +820,16 @@ void gemm_times_sd_add_times_sd(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W,
 
 template <typename T>
 void gemm_fused_mvsdi_mvTmv(T alpha, const Mvec<T>* V, Mvec<T>* W, const SdMat<T>* C, T beta, SdMat<T>* D) {
+  const int k = W->nvec, m = C->ncols;
+  PHX_REQUIRE(C->nrows == k && m <= k, PHIST_INVALID_INPUT, "C is %dx%d, W has %d columns", C->nrows, m, k);
+  PHX_REQUIRE(V->nrows == W->nrows, PHIST_INVALID_INPUT, "V and W have different lengths");
+  PHX_REQUIRE(D->nrows == V->nvec && D->ncols == m, PHIST_INVALID_INPUT, "D is %dx%d, expected %dx%d", D->nrows,
+              D->ncols, V->nvec, m);
+  // one sweep: V == W (the self product of the new block) or V apart from W
+  const bool self = V->root == W->root && V->jofs == W->jofs && V->nvec == W->nvec;
+  if (self ? m == k && mix_gram<T>(2, T(1), W, C, T(0), W, nullptr, alpha, beta, D)
+           : mix_gram<T>(2, T(1), W, C, T(0), W, V, alpha, beta, D))
+    return;
   gemm_times_sd_inplace<T>(W, C);
   Mvec<T> wv = *W;
   wv.root = W->root;
@@ -695,6 +839,14 @@ void gemm_fused_mvsdi_mvTmv(T alpha, const Mvec<T>* V, Mvec<T>* W, const SdMat<T
 
 template <typename T>
 void gemm_fused_mvsd_mvTmv(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W, SdMat<T>* WtW) {
+  const int k = V->nvec, m = W->nvec;
+  PHX_REQUIRE(C->nrows == k && C->ncols == m, PHIST_INVALID_INPUT, "C is %dx%d, expected %dx%d", C->nrows, C->ncols, k,
+              m);
+  PHX_REQUIRE(WtW->nrows == m && WtW->ncols == m, PHIST_INVALID_INPUT, "WtW must be %d x %d", m, m);
+  PHX_REQUIRE(V->nrows == W->nrows, PHIST_INVALID_INPUT, "V and W have different lengths");
+  if (V->root == W->root)
+    PHX_REQUIRE(V->jofs + k <= W->jofs || W->jofs + m <= V->jofs, PHIST_INVALID_INPUT, "V and W overlap in memory");
+  if (mix_gram<T>(0, alpha, V, C, beta, W, nullptr, T(1), T(0), WtW)) return;
   gemm_times_sd<T>(alpha, V, C, beta, W);
   gemm_gram<T>(T(1), W, W, T(0), WtW);
 }

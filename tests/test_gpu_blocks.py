@@ -268,6 +268,65 @@ def test_fused_mvsd_and_mvsdi(be64):
         o.delete()
 
 
+def gram_bound(dtype, alpha, A, B, beta, C0):
+    return 8 * RTOL[dtype] * (abs(alpha) * np.abs(A).T @ np.abs(B) + abs(beta) * np.abs(C0)) + 1e-300
+
+
+@pytest.mark.parametrize("k,m", [(8, 8), (16, 16), (12, 20), (32, 32), (64, 32), (9, 31), (40, 40), (4, 8)])
+def test_fused_mvsd_mvTmv_shapes(be, k, m):
+    """W = alpha*V*C + beta*W and W'W in one sweep (8 <= k <= 64, 8 <= m <= 32), else two passes"""
+    n = 3001
+    rng = np.random.default_rng(k * 13 + m)
+    V, W0, C0 = rng.uniform(-1, 1, (n, k)), rng.uniform(-1, 1, (n, m)), rng.uniform(-1, 1, (k, m))
+    mp = be.map(n)
+    v, w = be.mvec(mp, k, V), be.mvec(mp, m, W0)
+    C = be.sdmat(k, m, C0)
+    WtW = be.sdmat(m, m, np.full((m, m), 7.0))
+    be.call("fused_mvsd_mvTmv", be.s(-0.5), v.h, C.h, be.s(1.5), w.h, WtW.h)
+    Vt, Wt, Ct = (a.astype(be.dtype).astype(float) for a in (V, W0, C0))
+    Wn = ol.times_sdmat(-0.5, Vt, Ct, 1.5, Wt)
+    got_w = w.get().astype(float)
+    assert rel(got_w, Wn) <= 4 * RTOL[be.dtype.type] * k ** 0.5
+    want = ol.gram(1.0, got_w, got_w, 0.0, np.zeros((m, m)))
+    assert np.all(np.abs(WtW.get().astype(float) - want) <= gram_bound(be.dtype.type, 1.0, got_w, got_w, 0.0, want))
+    for o in (v, w, C, WtW):
+        o.delete()
+
+
+@pytest.mark.parametrize("nv,k,m", [(8, 8, 8), (16, 16, 16), (5, 24, 12), (32, 32, 32), (40, 16, 16), (0, 16, 16),
+                                    (0, 24, 24), (0, 32, 32), (0, 24, 16), (0, 8, 8)])
+def test_fused_mvsdi_mvTmv_shapes(be, nv, k, m):
+    """W(:,1:m) = W*C in place and D = alpha*V'W + beta*D in one sweep; nv = 0 is the
+    self product V == W (one sweep when m == k)"""
+    n = 2777
+    rng = np.random.default_rng(nv * 100 + k * 7 + m)
+    W0, C0 = rng.uniform(-1, 1, (n, k)), rng.uniform(-1, 1, (k, m))
+    mp = be.map(n)
+    w = be.mvec(mp, k, W0)
+    C = be.sdmat(k, m, C0)
+    Wt, Ct = W0.astype(be.dtype).astype(float), C0.astype(be.dtype).astype(float)
+    self_ = nv == 0
+    if self_:
+        v, Vt, nvv = w, None, k
+    else:
+        V = rng.uniform(-1, 1, (n, nv))
+        v, Vt, nvv = be.mvec(mp, nv, V), V.astype(be.dtype).astype(float), nv
+    D0 = rng.uniform(-1, 1, (nvv, m))
+    D = be.sdmat(nvv, m, D0)
+    Dt = D0.astype(be.dtype).astype(float)
+    be.call("fused_mvsdi_mvTmv", be.s(2.0), v.h, w.h, C.h, be.s(0.5), D.h)
+    got_w = w.get().astype(float)
+    want_w = Wt.copy()
+    want_w[:, :m] = Wt @ Ct
+    assert rel(got_w, want_w) <= 4 * RTOL[be.dtype.type] * k ** 0.5
+    left = got_w if self_ else Vt
+    want = ol.gram(2.0, left, got_w[:, :m], 0.5, Dt)
+    bound = gram_bound(be.dtype.type, 2.0, left, got_w[:, :m], 0.5, Dt)
+    assert np.all(np.abs(D.get().astype(float) - want) <= bound)
+    for o in ((w, C, D) if self_ else (v, w, C, D)):
+        o.delete()
+
+
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16, 32])
 def test_chol_qr(be64, k):
     n = 3000
