@@ -78,6 +78,41 @@ int phist_amd_laplace3d_rowfunc(phist_gidx row, phist_lidx* nnz, phist_gidx* col
 int phist_amd_laplace3d_rowfunc_f(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
 int phist_amd_rowfunc_shared_ctor(void* arg, void** work);
 
+/* Heisenberg/XXZ chain of L spins in the sector of nup up spins
+   (drivers/matfuncs/SpinChainSZ.c:189-310, called by the reference driver
+   with {L, L/2, 0} and jz = jxy = 1): C(L, nup) rows, at most L+1 entries. */
+typedef struct {
+  int32_t L;
+  int32_t nup;
+  int32_t obc; /* 0: periodic chain, 1: open */
+  int32_t pad_;
+  double jz;
+  double jxy;
+} phist_amd_spinchain_sz_params;
+void phist_amd_spinchain_sz_dims(const phist_amd_spinchain_sz_params* p, phist_gidx* nrows, phist_lidx* maxnne,
+                                 int* iflag);
+int phist_amd_spinchain_sz_rowfunc(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+int phist_amd_spinchain_sz_rowfunc_f(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+
+/* tight-binding honeycomb (graphene) sheet of W x L sites
+   (drivers/matfuncs/graphene.c:38-245): nearest neighbours -1, with
+   long_range also -0.1/2.78 (second) and -0.095/2.78 (third neighbours);
+   on-site gamma*(u-1/2) with u from a per-row seeded stream. */
+typedef struct {
+  int64_t W;
+  int64_t L;
+  int32_t pbc_w;
+  int32_t pbc_l;
+  int32_t long_range;
+  int32_t pad_;
+  double gamma;
+  uint64_t seed;
+} phist_amd_honeycomb_params;
+void phist_amd_honeycomb_dims(const phist_amd_honeycomb_params* p, phist_gidx* nrows, phist_lidx* maxnne,
+                              int* iflag);
+int phist_amd_honeycomb_rowfunc(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+int phist_amd_honeycomb_rowfunc_f(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
+
 #ifdef __cplusplus
 }
 #endif

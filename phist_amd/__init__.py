@@ -50,6 +50,42 @@ class Laplace3dParams(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64)]
 
 
+class SpinSzParams(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int32), ("nup", ctypes.c_int32), ("obc", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("jz", ctypes.c_double), ("jxy", ctypes.c_double)]
+
+
+class HoneycombParams(ctypes.Structure):
+    _fields_ = [("W", ctypes.c_int64), ("L", ctypes.c_int64), ("pbc_w", ctypes.c_int32), ("pbc_l", ctypes.c_int32),
+                ("long_range", ctypes.c_int32), ("pad_", ctypes.c_int32), ("gamma", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+def generator_dims(kind, params):
+    """(nrows, max entries per row) of a library generator ('spin_sz' or 'honeycomb')"""
+    n, mx, fl = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int(0)
+    getattr(lib(), f"phist_amd_{'spinchain_sz' if kind == 'spin_sz' else kind}_dims")(
+        ctypes.byref(params), ctypes.byref(n), ctypes.byref(mx), ctypes.byref(fl))
+    if fl.value:
+        raise PhistError(f"{kind}_dims", fl.value)
+    return n.value, mx.value
+
+
+def generator_row(kind, params, row, dtype=np.float64):
+    """one row (cols, vals) of a library generator, computed on the host"""
+    n, mx = generator_dims(kind, params)
+    cols = np.zeros(mx, np.int64)
+    vals = np.zeros(mx, dtype)
+    k = ctypes.c_int32(0)
+    name = {"spin_sz": "spinchain_sz", "honeycomb": "honeycomb"}[kind]
+    f = getattr(lib(), f"phist_amd_{name}_rowfunc" + ("" if dtype == np.float64 else "_f"))
+    rc = f(ctypes.c_int64(row), ctypes.byref(k), cols.ctypes.data_as(ctypes.c_void_p),
+           vals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(params))
+    if rc != 0:
+        raise PhistError(f"{name}_rowfunc", rc)
+    return cols[:k.value].copy(), vals[:k.value].copy()
+
+
 _lib = None
 
 
@@ -155,6 +191,20 @@ class Backend:
         self._lap = Laplace3dParams(nx, ny, nz)
         f = self.L.phist_amd_laplace3d_rowfunc if self.p == "D" else self.L.phist_amd_laplace3d_rowfunc_f
         return self.from_rowfunc(nx * ny * nz, 7, f, ctypes.byref(self._lap), parallel=True)
+
+    def spinchain_sz(self, L, nup=None, obc=0, jz=1.0, jxy=1.0):
+        """XXZ/Heisenberg chain of L spins in the sector of nup up spins (default L/2)"""
+        self._spin = SpinSzParams(L, L // 2 if nup is None else nup, obc, 0, jz, jxy)
+        n, mx = generator_dims("spin_sz", self._spin)
+        f = self.L.phist_amd_spinchain_sz_rowfunc if self.p == "D" else self.L.phist_amd_spinchain_sz_rowfunc_f
+        return self.from_rowfunc(n, mx, f, ctypes.byref(self._spin), parallel=True)
+
+    def honeycomb(self, W, L, pbc_w=0, pbc_l=0, long_range=0, gamma=0.0, seed=0):
+        """tight-binding honeycomb sheet of W x L sites"""
+        self._honey = HoneycombParams(W, L, pbc_w, pbc_l, long_range, 0, gamma, seed)
+        n, mx = generator_dims("honeycomb", self._honey)
+        f = self.L.phist_amd_honeycomb_rowfunc if self.p == "D" else self.L.phist_amd_honeycomb_rowfunc_f
+        return self.from_rowfunc(n, mx, f, ctypes.byref(self._honey), parallel=True)
 
     # -- kernels (thin) ------------------------------------------------------
     def spmv(self, alpha, A, x, beta, y):

@@ -33,8 +33,8 @@ _oracle = None
 def oracle():
     global _oracle
     if _oracle is None:
-        if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(
-                os.path.join(ORACLE_DIR, "builtin_oracle.c")):
+        srcs = [os.path.join(ORACLE_DIR, f) for f in ("builtin_oracle.c", "matfuncs_oracle.c")]
+        if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < max(map(os.path.getmtime, srcs)):
             subprocess.check_call(["make", "-s", "-C", ORACLE_DIR, "all"])
         _oracle = ctypes.CDLL(ORACLE_SO)
         _oracle.oracle_kiss_raw.restype = ctypes.c_uint64
@@ -196,3 +196,47 @@ def mvec_random_sequence(shapes):
         o.oracle_mvec_random(ctypes.byref(s), ctypes.c_int64(n), ctypes.c_int(k), p(v), ctypes.c_int64(k))
         outs.append(v)
     return outs
+
+
+# ---- problem generators (drivers/matfuncs restatements) ----
+
+def spin_sz_rows(L, nup, obc=0):
+    """rows of the S_z-sector spin chain from the oracle: list of (cols, vals)"""
+    o = oracle()
+    o.oracle_spin_sz_init.restype = ctypes.c_void_p
+    n = ctypes.c_int64(0)
+    h = o.oracle_spin_sz_init(ctypes.c_int(L), ctypes.c_int(nup), ctypes.c_int(obc), ctypes.byref(n))
+    cols = np.zeros(L + 1, np.int64)
+    vals = np.zeros(L + 1)
+    k = ctypes.c_int(0)
+    rows = []
+    for r in range(n.value):
+        rc = o.oracle_spin_sz_row(ctypes.c_void_p(h), ctypes.c_int64(r), ctypes.byref(k), p(cols), p(vals))
+        assert rc == 0
+        rows.append((cols[:k.value].copy(), vals[:k.value].copy()))
+    o.oracle_spin_sz_free(ctypes.c_void_p(h))
+    return rows
+
+
+def honeycomb_rows(W, L, pbc_w=0, pbc_l=0, long_range=0, onsite=None):
+    """rows of the honeycomb sheet from the oracle; onsite[r] is the disorder part of row r"""
+    o = oracle()
+    cols = np.zeros(16, np.int64)
+    vals = np.zeros(16)
+    k = ctypes.c_int(0)
+    rows = []
+    for r in range(W * L):
+        v = 0.0 if onsite is None else float(onsite[r])
+        o.oracle_honeycomb_row(ctypes.c_int64(W), ctypes.c_int64(L), ctypes.c_int(pbc_w), ctypes.c_int(pbc_l),
+                               ctypes.c_int(long_range), ctypes.c_double(v), ctypes.c_int64(r), ctypes.byref(k),
+                               p(cols), p(vals))
+        rows.append((cols[:k.value].copy(), vals[:k.value].copy()))
+    return rows
+
+
+def rows_to_dense(rows, n):
+    A = np.zeros((n, n))
+    for r, (c, v) in enumerate(rows):
+        for cc, vv in zip(c, v):
+            A[r, cc] += vv
+    return A
