@@ -83,6 +83,22 @@ inline int persistent_blocks(K kernel, size_t lds_bytes, int64_t ntiles, int max
   return (int)g;
 }
 
+// Adds the partial tiles of the four waves of a workgroup in LDS (wave 0,
+// then 1, 2, 3) and stores the sum as the workgroup's tile out[0..nout).
+// put(add) is called once per wave; each lane passes the elements (o, v) it
+// owns to add(o, v), and every element of the tile is owned by exactly one
+// lane of each wave.  sh holds nout elements (dynamic LDS).
+template <typename T, typename F>
+__device__ inline void block_tile_sum(T* sh, int nout, T* __restrict__ out, F&& put) {
+  const int wv = threadIdx.x >> 6;
+#pragma unroll 1
+  for (int w = 0; w < kBlock / 64; w++) {
+    if (wv == w) put([&](int o, T v) { sh[o] = w == 0 ? v : sh[o] + v; });
+    __syncthreads();
+  }
+  for (int o = threadIdx.x; o < nout; o += kBlock) out[o] = sh[o];
+}
+
 // Sum over b of partials[b*width + o] for the 16 outputs o = 16*blockIdx.x +
 // (tid & 15) of this workgroup: thread group p = tid / 16 adds b = p, p+16, ...
 // in order, then the 16 group sums are added in order of p.  The result is

@@ -37,7 +37,7 @@ struct GramArgs {
   const T* __restrict__ b;  // broadcast operand (width nb <= LPR*VPL)
   int64_t ldb;
   int nb;
-  T* __restrict__ partials;  // [grid*4 waves][na*nb], index c*na + r (r over a, c over b)
+  T* __restrict__ partials;  // [grid][na*nb], index c*na + r (r over a, c over b)
 };
 
 template <typename T, int VPL, int LPR, int KB>
@@ -75,16 +75,19 @@ __global__ void __launch_bounds__(kBlock) k_gram(const GramArgs<T> g) {
     }
   }
 
-  // reduce over the rows of the wave; one partial tile per wave
+  // reduce over the rows of the wave, then over the waves: one partial tile
+  // per workgroup
+  extern __shared__ char gram_lds[];
   const int nout = g.na * g.nb;
-  T* part = g.partials + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * nout;
+  block_tile_sum(reinterpret_cast<T*>(gram_lds), nout, g.partials + (int64_t)blockIdx.x * nout, [&](auto add) {
 #pragma unroll
-  for (int e = 0; e < VPL; e++)
+    for (int e = 0; e < VPL; e++)
 #pragma unroll
-    for (int c = 0; c < KB; c++) {
-      const T s = wave_sum_strided<LPR>(acc[e][c]);
-      if (lane < LPR && c0 + e < g.na && c < g.nb) part[c * g.na + c0 + e] = s;
-    }
+      for (int c = 0; c < KB; c++) {
+        const T s = wave_sum_strided<LPR>(acc[e][c]);
+        if (lane < LPR && c0 + e < g.na && c < g.nb) add(c * g.na + c0 + e, s);
+      }
+  });
 }
 
 // ---- MFMA Gram (both operands at least 16 columns) ----
@@ -151,21 +154,23 @@ __global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
           acc[tm][tk] = M::op(a[st][tm], SELF ? a[st][tk] : b[st][tk], acc[tm][tk]);
         }
   }
+  extern __shared__ char gram_lds[];
   const int nout = g.na * g.nb;
-  T* part = g.partials + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * nout;
+  block_tile_sum(reinterpret_cast<T*>(gram_lds), nout, g.partials + (int64_t)blockIdx.x * nout, [&](auto add) {
 #pragma unroll
-  for (int tm = 0; tm < TM; tm++)
+    for (int tm = 0; tm < TM; tm++)
 #pragma unroll
-    for (int tk = 0; tk < TK; tk++) {
-      if (SELF && tm > tk) continue;
+      for (int tk = 0; tk < TK; tk++) {
+        if (SELF && tm > tk) continue;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = tm * 16 + M::row(lane, r), j = tk * 16 + cc;
-        const T v = acc[tm][tk][r];
-        if (i < g.na && j < g.nb) part[j * g.na + i] = v;
-        if (SELF && tm < tk && j < g.na && i < g.nb) part[i * g.na + j] = v;
+        for (int r = 0; r < 4; r++) {
+          const int i = tm * 16 + M::row(lane, r), j = tk * 16 + cc;
+          const T v = acc[tm][tk][r];
+          if (i < g.na && j < g.nb) add(j * g.na + i, v);
+          if (SELF && tm < tk && j < g.na && i < g.nb) add(i * g.na + j, v);
+        }
       }
-    }
+  });
 }
 
 // C(r,c) = alpha * S + beta * C with S summed over the block partials in a
@@ -194,13 +199,14 @@ int gram_launch_one(GramArgs<T>& g) {
   auto kern = k_gram<T, VPL, LPR, KB>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   g.ntiles = (g.nrows + RPB - 1) / RPB;
-  const int grid = persistent_blocks(kern, 0, g.ntiles, rt().gram_blocks_per_cu);
-  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * g.na * g.nb);
+  const size_t lds = sizeof(T) * g.na * g.nb;
+  const int grid = persistent_blocks(kern, lds, g.ntiles, rt().gram_blocks_per_cu);
+  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * g.na * g.nb);
   ktimer_begin();
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, g);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, g);
   PHX_HIP(hipGetLastError());
   ktimer_end();
-  return grid * (kBlock / 64);
+  return grid;
 }
 
 template <typename T, int VPL, int KB>
@@ -221,13 +227,14 @@ template <typename T, int TM, int TK, bool SELF>
 static int gram_mfma_launch(GramArgs<T>& g) {
   auto kern = k_gram_mfma<T, TM, TK, SELF>;
   g.ntiles = (g.nrows + 63) / 64;
-  const int grid = persistent_blocks(kern, 0, g.ntiles, rt().gram_blocks_per_cu);
-  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * g.na * g.nb);
+  const size_t lds = sizeof(T) * g.na * g.nb;
+  const int grid = persistent_blocks(kern, lds, g.ntiles, rt().gram_blocks_per_cu);
+  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * g.na * g.nb);
   ktimer_begin();
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, g);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, g);
   PHX_HIP(hipGetLastError());
   ktimer_end();
-  return grid * (kBlock / 64);
+  return grid;
 }
 
 // S = a'b on the matrix cores for na, nb <= 32 (zero-padded to 16-column
@@ -287,7 +294,7 @@ static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int
     const int groups = (std::max(g.na, nb) + VPL - 1) / VPL;
     int lpr = 1;
     while (lpr < groups) lpr <<= 1;
-    int nparts;  // partial tiles written (one per wave)
+    int nparts;  // partial tiles written (one per workgroup)
     if (nb <= 8) {
       nparts = VPL == 1 ? gram_lpr<T, 1, 8>(g, lpr) : gram_lpr<T, V16, 8>(g, lpr);
     } else {
@@ -556,22 +563,25 @@ __global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
     }
   }
   if constexpr (GRAM != 0) {
-    // partial tile of this wave: S(i, j) at j*na + i
+    // partial tile of this workgroup: S(i, j) at j*na + i
+    extern __shared__ char gram_lds[];
     const int na = GRAM == 1 ? p.m : p.ngv, nb = p.m;
-    T* part = p.partials + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * na * nb;
+    block_tile_sum(reinterpret_cast<T*>(gram_lds), na * nb, p.partials + (int64_t)blockIdx.x * na * nb,
+                   [&](auto add) {
 #pragma unroll
-    for (int tg = 0; tg < TG; tg++)
+                     for (int tg = 0; tg < TG; tg++)
 #pragma unroll
-      for (int tm = 0; tm < TM; tm++) {
-        if (GRAM == 1 && tg > tm) continue;
+                       for (int tm = 0; tm < TM; tm++) {
+                         if (GRAM == 1 && tg > tm) continue;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = tg * 16 + M::row(lane, r), j = tm * 16 + cc;
-          const T v = gacc[tg][tm][r];
-          if (i < na && j < nb) part[j * na + i] = v;
-          if (GRAM == 1 && tg < tm && j < na && i < nb) part[i * na + j] = v;
-        }
-      }
+                         for (int r = 0; r < 4; r++) {
+                           const int i = tg * 16 + M::row(lane, r), j = tm * 16 + cc;
+                           const T v = gacc[tg][tm][r];
+                           if (i < na && j < nb) add(j * na + i, v);
+                           if (GRAM == 1 && tg < tm && j < na && i < nb) add(i * na + j, v);
+                         }
+                       }
+                   });
   }
 }
 
@@ -632,14 +642,15 @@ int mix_gram_launch(MixArgs<T>& p) {
   constexpr int RT = KS >= 8 ? 1 : 2;
   auto kern = k_rowmix_mfma<T, KS, TM, RT, GRAM, TV>;
   p.ntiles = (p.nrows + 64 * RT - 1) / (64 * RT);
-  const int grid = persistent_blocks(kern, 0, p.ntiles, rt().gram_blocks_per_cu);
   const int na = GRAM == 1 ? p.m : p.ngv;
-  p.partials = (T*)device_work(sizeof(T) * (size_t)grid * (kBlock / 64) * na * p.m);
+  const size_t lds = sizeof(T) * na * p.m;
+  const int grid = persistent_blocks(kern, lds, p.ntiles, rt().gram_blocks_per_cu);
+  p.partials = (T*)device_work(sizeof(T) * (size_t)grid * na * p.m);
   ktimer_begin();
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, p);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, p);
   PHX_HIP(hipGetLastError());
   ktimer_end();
-  return grid * (kBlock / 64);
+  return grid;
 }
 
 template <typename T, int KS, int TM>
