@@ -179,6 +179,54 @@ def tall_skinny(be, args):
     print(json.dumps(out), flush=True)
 
 
+def jada_bench(be, args):
+    """Time to 1e-8 of the symmetric block Jacobi-Davidson, FP64 throughout vs
+    FP32 correction equations (outer iteration FP64), same start vector."""
+    prob = args.problem
+    t0 = time.perf_counter()
+    if prob.startswith("xxz"):
+        L = int(prob[3:])
+        A = be.spinchain_sz(L, L // 2)
+        cfg = dict(num_eigs=16, block_size=4, min_bas=24, max_bas=48, inner_max_iters=10)
+        name = f"XXZ chain L={L} S_z=0 ({A.nrows} rows, {A.nnz} nnz), 16 smallest"
+    else:
+        n = int(prob[3:])
+        A = be.laplace3d(n, n, n)
+        cfg = dict(num_eigs=8, block_size=4, min_bas=16, max_bas=32, inner_max_iters=10)
+        name = f"3D Laplacian {n}^3 ({A.nrows} rows), 8 smallest"
+    t_build = time.perf_counter() - t0
+    print(f"bench: {name}; built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    results = {}
+    for label, fp32 in (("fp64", False), ("mixed", True)):
+        iflag = ctypes.c_int(0)
+        be.L.phist_amd_kiss_reset(ctypes.byref(iflag))  # identical random start vector for both runs
+        if fp32:  # the FP32 copy of A is made once per matrix: build it outside the timed run
+            b, x = be.mvec(A.domain_map, 1), be.mvec(A.domain_map, 1)
+            be.minres_shifted(A, [0.0], b, x, 1.0, 1, fp32=True)
+            b.delete()
+            x.delete()
+        be.synchronize()
+        t0 = time.perf_counter()
+        ev, res, nconv, niter, Q, R = be.subspacejada(A, which="SR", tol=1e-8, max_iters=500, inner_fp32=fp32,
+                                                      **cfg)
+        be.synchronize()
+        dt = time.perf_counter() - t0
+        results[label] = {"seconds": round(dt, 3), "outer_iters": niter, "converged": nconv,
+                          "lowest": round(float(np.min(ev[:nconv])) if nconv else float("nan"), 10),
+                          "max_res": float(np.max(res[:cfg["num_eigs"]]))}
+        print(f"bench: jada {label}: {dt:.2f} s, {niter} iterations, {nconv} converged", file=sys.stderr, flush=True)
+        Q.delete()
+        R.delete()
+    speed = results["fp64"]["seconds"] / results["mixed"]["seconds"]
+    out = {"metric": "JD time-to-1e-8 mixed vs FP64", "value": results["mixed"]["seconds"], "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f64 outer / f32 inner",
+           "data": "synthetic (generated matrix, KISS random start vector)",
+           "config": {"workload": name, "which": "SR", "tol": 1e-8, **cfg},
+           "fp64": results["fp64"], "mixed": results["mixed"], "speedup_mixed_vs_fp64": round(speed, 3)}
+    print(json.dumps(out), flush=True)
+    A.delete()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,11 +235,13 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd"], default="spmv",
+    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada"], default="spmv",
                     help="spmv (default): the fused SpMV line; gram / cholqr / mvsd (fused_mvsd_mvTmv, "
                          "W = V*C and W'W, beta=0): the tall-skinny configuration (16,777,216 x k, U(-1,1)) "
                          "reported in the same format")
     ap.add_argument("--k", type=int, default=8, help="block width for --op gram/cholqr")
+    ap.add_argument("--problem", default="xxz24",
+                    help="--op jada: xxz<L> (S_z=0 spin chain, 16 smallest) or lap<n> (n^3 Laplacian, 8 smallest)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the library's STREAM copy kernel (1 GiB in, 1 GiB out per launch) so a "
                          "counter profile of this command holds a known-size reference dispatch")
@@ -206,6 +256,8 @@ def main():
     be.init()
     L = be.L
     iflag = ctypes.c_int(0)
+    if args.op == "jada":
+        return jada_bench(be, args)
     if args.op != "spmv":
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))

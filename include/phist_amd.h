@@ -95,7 +95,7 @@ typedef const phist_DsdMat* phist_Dconst_sdMat_ptr;
 typedef struct phist_DsparseMat_s phist_DsparseMat;
 typedef phist_DsparseMat* phist_DsparseMat_ptr;
 typedef const phist_DsparseMat* phist_Dconst_sparseMat_ptr;
-typedef struct phist_DlinearOp_s phist_DlinearOp;
+typedef struct phist_DlinearOp phist_DlinearOp; /* defined in phist_amd_jada.h */
 typedef const phist_DlinearOp* phist_Dconst_linearOp_ptr;
 
 typedef struct phist_Smvec_s phist_Smvec;
@@ -107,7 +107,7 @@ typedef const phist_SsdMat* phist_Sconst_sdMat_ptr;
 typedef struct phist_SsparseMat_s phist_SsparseMat;
 typedef phist_SsparseMat* phist_SsparseMat_ptr;
 typedef const phist_SsparseMat* phist_Sconst_sparseMat_ptr;
-typedef struct phist_SlinearOp_s phist_SlinearOp;
+typedef struct phist_SlinearOp phist_SlinearOp; /* defined in phist_amd_jada.h */
 typedef const phist_SlinearOp* phist_Sconst_linearOp_ptr;
 
 /* row and element callbacks (phist_kernels.h:197-202) */

@@ -86,6 +86,36 @@ def generator_row(kind, params, row, dtype=np.float64):
     return cols[:k.value].copy(), vals[:k.value].copy()
 
 
+class JadaOpts(ctypes.Structure):
+    """phist_jadaOpts (src/jada/phist_jadaOpts.h:33-199)"""
+    _fields_ = [("numEigs", ctypes.c_int), ("which", ctypes.c_int), ("convTol", ctypes.c_double),
+                ("relConvTol", ctypes.c_int), ("symmetry", ctypes.c_int), ("how", ctypes.c_int),
+                ("maxIters", ctypes.c_int), ("blockSize", ctypes.c_int), ("lookAhead", ctypes.c_int),
+                ("minBas", ctypes.c_int), ("maxBas", ctypes.c_int), ("v0", ctypes.c_void_p), ("arno", ctypes.c_int),
+                ("initialShift_r", ctypes.c_double), ("initialShift_i", ctypes.c_double),
+                ("initialShiftIters", ctypes.c_int), ("innerSolvType", ctypes.c_int),
+                ("innerSolvBlockSize", ctypes.c_int), ("innerSolvMaxBas", ctypes.c_int),
+                ("innerSolvMaxIters", ctypes.c_int), ("innerSolvBaseTol", ctypes.c_double),
+                ("innerSolvRobust", ctypes.c_int), ("innerSolvStopAfterFirstConverged", ctypes.c_int),
+                ("innerSolvMaxProjectionSpace", ctypes.c_int), ("preconOp", ctypes.c_void_p),
+                ("preconType", ctypes.c_int), ("preconOpts", ctypes.c_char * 1024),
+                ("preconSkewProject", ctypes.c_int), ("preconUpdate", ctypes.c_int),
+                ("customSolver", ctypes.c_void_p), ("customSolver_run", ctypes.c_void_p),
+                ("customSolver_run1", ctypes.c_void_p)]
+
+
+class LinearOp(ctypes.Structure):
+    """phist_[DS]linearOp (src/core/phist_operator_decl.h:14-74); function members as raw pointers"""
+    _fields_ = [("A", ctypes.c_void_p), ("aux", ctypes.c_void_p), ("range_map", ctypes.c_void_p),
+                ("domain_map", ctypes.c_void_p), ("use_transpose", ctypes.c_int), ("shifts", ctypes.c_void_p),
+                ("apply", ctypes.c_void_p), ("applyT", ctypes.c_void_p), ("apply_shifted", ctypes.c_void_p),
+                ("fused_apply_mvTmv", ctypes.c_void_p), ("update", ctypes.c_void_p), ("destroy", ctypes.c_void_p)]
+
+
+EIGSORT = {"LM": 1, "SM": 2, "LR": 3, "SR": 4}
+LINSOLV = {"none": 0, "GMRES": 1, "MINRES": 2}
+HERMITIAN = 1
+
 _lib = None
 
 
@@ -205,6 +235,66 @@ class Backend:
         n, mx = generator_dims("honeycomb", self._honey)
         f = self.L.phist_amd_honeycomb_rowfunc if self.p == "D" else self.L.phist_amd_honeycomb_rowfunc_f
         return self.from_rowfunc(n, mx, f, ctypes.byref(self._honey), parallel=True)
+
+    # -- solvers -------------------------------------------------------------
+    def subspacejada(self, A, num_eigs, which="SR", tol=1e-8, block_size=1, min_bas=None, max_bas=None,
+                     max_iters=300, inner="MINRES", inner_max_iters=20, inner_base_tol=0.1, inner_fp32=False,
+                     rel_tol=0, v0=None, lookahead=-1):
+        """phist_Xsubspacejada on a wrapped sparse matrix (symmetric, standard).
+        Returns (ev, resNorm, nConv, nIter, Q, R) with Q an Mvec and R an SdMat."""
+        opts = JadaOpts()
+        self.L.phist_jadaOpts_setDefaults(ctypes.byref(opts))
+        opts.numEigs = num_eigs
+        opts.which = EIGSORT[which]
+        opts.convTol = tol
+        opts.relConvTol = rel_tol
+        opts.symmetry = HERMITIAN
+        opts.blockSize = block_size
+        opts.minBas = min_bas if min_bas is not None else max(10, num_eigs + block_size)
+        opts.maxBas = max_bas if max_bas is not None else opts.minBas + 2 * block_size + 10
+        opts.maxIters = max_iters
+        opts.lookAhead = lookahead
+        opts.innerSolvType = LINSOLV[inner]
+        opts.innerSolvMaxIters = inner_max_iters
+        opts.innerSolvBaseTol = inner_base_tol
+        if v0 is not None:
+            opts.v0 = v0.h
+        op = LinearOp()
+        fl = ctypes.c_int(0)
+        getattr(self.L, f"phist_{self.p}linearOp_wrap_sparseMat")(ctypes.byref(op), A.h, ctypes.byref(fl))
+        if fl.value:
+            raise PhistError("linearOp_wrap_sparseMat", fl.value)
+        self.L.phist_amd_jada_set_inner_fp32(1 if inner_fp32 else 0, ctypes.byref(fl))
+        nq = num_eigs + block_size - 1
+        Q = self.mvec(A.domain_map, nq)
+        R = self.sdmat(nq, nq)
+        ev = np.zeros(nq, self.dtype)
+        res = np.zeros(num_eigs, self.dtype)
+        nconv, niter = ctypes.c_int(0), ctypes.c_int(0)
+        fl = ctypes.c_int(0)
+        getattr(self.L, f"phist_{self.p}subspacejada")(ctypes.byref(op), None, opts, Q.h, R.h, _ptr(ev), _ptr(res),
+                                                       ctypes.byref(nconv), ctypes.byref(niter), ctypes.byref(fl))
+        self.L.phist_amd_jada_set_inner_fp32(0, ctypes.byref(ctypes.c_int(0)))
+        if fl.value < 0:
+            Q.delete()
+            R.delete()
+            raise PhistError("subspacejada", fl.value)
+        return ev, res, nconv.value, niter.value, Q, R
+
+    def minres_shifted(self, A, shifts, B, X, tol, max_iter, fp32=False):
+        """phist_amd_DblockedMINRES_shifted: (A + shifts[j] I) x_j = b_j; returns (iters, relres, unconverged)"""
+        k = len(shifts)
+        sh = np.ascontiguousarray(shifts, np.float64)
+        tl = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, np.float64), (k,)))
+        it = np.zeros(k, np.int32)
+        rr = np.zeros(k)
+        fl = ctypes.c_int(0)
+        self.L.phist_amd_DblockedMINRES_shifted(A.h, ctypes.c_int(k), _ptr(sh), B.h, X.h, _ptr(tl),
+                                                ctypes.c_int(max_iter), ctypes.c_int(1 if fp32 else 0), _ptr(it),
+                                                _ptr(rr), ctypes.byref(fl))
+        if fl.value < 0:
+            raise PhistError("blockedMINRES_shifted", fl.value)
+        return it, rr, fl.value
 
     # -- kernels (thin) ------------------------------------------------------
     def spmv(self, alpha, A, x, beta, y):

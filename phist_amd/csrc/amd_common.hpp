@@ -203,7 +203,13 @@ struct SparseMat {
   std::vector<int64_t> h_chunk_ptr;
   std::vector<int32_t> h_chunk_w;
   std::vector<int32_t> h_perm;
+  // FP32 copy of an FP64 matrix for mixed-precision solvers (made on first
+  // use, freed with the matrix)
+  SparseMat<float>* fp32_twin = nullptr;
 };
+
+// the cached FP32 copy of A (same SELL layout, values rounded to float)
+const SparseMat<float>* sparse_fp32_twin(const SparseMat<double>* A);
 
 // build the SELL-C-sigma device copy of a matrix whose host CRS is filled
 template <typename T> void sell_build(SparseMat<T>* A, int C, int sigma);

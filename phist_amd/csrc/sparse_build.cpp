@@ -172,6 +172,7 @@ template <typename T>
 void sparse_free(SparseMat<T>* A) {
   if (!A) return;
   sync_stream();
+  if (A->fp32_twin) sparse_free(A->fp32_twin);
   if (A->d_chunk_ptr) PHX_HIP(hipFree(A->d_chunk_ptr));
   if (A->d_chunk_w) PHX_HIP(hipFree(A->d_chunk_w));
   if (A->d_perm) PHX_HIP(hipFree(A->d_perm));
@@ -181,6 +182,30 @@ void sparse_free(SparseMat<T>* A) {
   if (A->domain_map && A->domain_map != A->row_map && A->domain_map != A->col_map) delete A->domain_map;
   delete A->row_map;
   delete A;
+}
+
+const SparseMat<float>* sparse_fp32_twin(const SparseMat<double>* A) {
+  if (A->fp32_twin) return A->fp32_twin;
+  auto* F = new SparseMat<float>();
+  F->nrows = A->nrows;
+  F->ncols = A->ncols;
+  F->nnz = A->nnz;
+  F->max_row_len = A->max_row_len;
+  F->row_map = new Map(*A->row_map);
+  F->col_map = A->col_map == A->row_map ? F->row_map : new Map(*A->col_map);
+  F->range_map = F->row_map;
+  F->domain_map = F->col_map;
+  F->ctx.row_map = F->row_map;
+  F->ctx.col_map = F->col_map;
+  F->ctx.range_map = F->range_map;
+  F->ctx.domain_map = F->domain_map;
+  F->crs_ptr = A->crs_ptr;
+  F->crs_col = A->crs_col;
+  F->crs_val.resize(A->crs_val.size());
+  for (size_t i = 0; i < A->crs_val.size(); i++) F->crs_val[i] = (float)A->crs_val[i];
+  sell_build(F, A->C, A->sigma);
+  const_cast<SparseMat<double>*>(A)->fp32_twin = F;
+  return F;
 }
 
 // ---- construction entry points ----

@@ -1,0 +1,157 @@
+"""GPU: blocked shifted MINRES and the symmetric block Jacobi-Davidson.
+
+Pins: MINRES solutions by their true residual ||b - (A + s I) x|| computed on
+the host from the oracle's CRS product; eigenvalues of the 3D Dirichlet
+Laplacian against the closed form sum_d (2 - 2 cos(pi i_d / (n+1))); spin
+chain eigenvalues against dense diagonalisation of the oracle's matrix; the
+returned partial Schur form by ||A Q - Q R|| and Q'Q = I (the invariants
+test/jada/TestSubspaceJaDa_def.hpp checks).  The FP32-inner (mixed) runs must
+reach the same FP64 tolerance.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def laplace_eigs(nx, ny, nz, k):
+    lam = [[2 - 2 * np.cos(np.pi * i / (n + 1)) for i in range(1, n + 1)] for n in (nx, ny, nz)]
+    ev = sorted(a + b + c for a, b, c in itertools.product(*lam))
+    return np.array(ev[:k])
+
+
+def crs_of(A):
+    ro, ci, va = A.crs()
+    return ol.Crs(A.nrows, A.ncols, ro, ci, va.astype(float))
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_minres_shifted_laplace(be64, fp32):
+    n = 12
+    A = be64.laplace3d(n, n, n)
+    N = n ** 3
+    rng = np.random.default_rng(5)
+    shifts = np.array([0.0, -0.5, -1.3, 2.0, -3.0, 0.7, -6.5, 1.0])  # several indefinite
+    k = len(shifts)
+    Bh = rng.uniform(-1, 1, (N, k))
+    m = be64.map(N)
+    B, X = be64.mvec(m, k, Bh), be64.mvec(m, k)
+    tol = 1e-5 if fp32 else 1e-10
+    it, rr, unconv = be64.minres_shifted(A, shifts, B, X, tol, 2000, fp32=fp32)
+    assert unconv == 0, (it, rr)
+    Xh = X.get()
+    AX = ol.spmv(crs_of(A), 1.0, Xh, 0.0, np.zeros((N, k)))
+    true = np.linalg.norm(Bh - AX - shifts * Xh, axis=0) / np.linalg.norm(Bh, axis=0)
+    assert np.all(true <= (20 * tol if fp32 else 1e-8)), true
+    assert np.all(rr <= tol)
+    for o in (B, X, A):
+        o.delete()
+
+
+def check_found(got, w, which, tol=1e-7):
+    """every converged Ritz value is an eigenvalue, and the distinct values
+    found are the first distinct eigenvalues in `which` order (a search space
+    grown from one vector may hold fewer copies of a multiple eigenvalue)"""
+    got = np.asarray(got, float)
+    assert np.min(np.abs(got[:, None] - w[None, :]), axis=1).max() <= tol, (got, w)
+    key = {"SR": lambda x: x, "LR": lambda x: -x, "LM": lambda x: -np.abs(x), "SM": np.abs}[which]
+    dw = []
+    for x in sorted(w, key=key):
+        if not dw or abs(x - dw[-1]) > 1e-6:
+            dw.append(x)
+    dg = []
+    for x in sorted(got, key=key):
+        if not dg or abs(x - dg[-1]) > 1e-6:
+            dg.append(x)
+    assert np.allclose(dg, dw[:len(dg)], atol=tol), (dg, dw[:len(dg) + 2])
+
+
+def check_schur(be, A, Q, R, nconv, tol):
+    Qh = Q.get()[:, :nconv]
+    Rh = R.get()[:nconv, :nconv]
+    AQ = ol.spmv(crs_of(A), 1.0, Qh, 0.0, np.zeros_like(Qh))
+    res = np.linalg.norm(AQ - Qh @ Rh, axis=0)
+    assert np.all(res <= 10 * tol), res
+    assert np.allclose(Qh.T @ Qh, np.eye(nconv), atol=1e-10)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("bs", [1, 2, 4])
+def test_jada_laplace_smallest(be64, bs, fp32):
+    """box of incommensurate sides: the smallest eigenvalues are simple (a
+    Krylov space grown from one start vector holds one copy of a multiple
+    eigenvalue in exact arithmetic, so a cube is not a fair test)"""
+    nx, ny, nz = 9, 11, 13
+    A = be64.laplace3d(nx, ny, nz)
+    k = 6
+    tol = 1e-8
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, "SR", tol, block_size=bs, min_bas=12 + bs,
+                                                    max_bas=24 + 2 * bs, max_iters=400, inner_max_iters=20,
+                                                    inner_fp32=fp32)
+    assert nconv >= k, (nconv, niter, res)
+    want = laplace_eigs(nx, ny, nz, k)
+    assert np.allclose(np.sort(ev[:k]), want, rtol=0, atol=1e-7), (np.sort(ev[:k]), want)
+    assert np.all(res[:k] <= tol)
+    check_schur(be64, A, Q, R, k, tol)
+    for o in (Q, R, A):
+        o.delete()
+
+
+@pytest.mark.parametrize("which", ["SR", "LR", "LM"])
+def test_jada_spin_chain(be64, which):
+    L = 10
+    A = be64.spinchain_sz(L, L // 2)
+    dense = ol.rows_to_dense(ol.spin_sz_rows(L, L // 2, 0), A.nrows)
+    w = np.linalg.eigvalsh(dense)
+    k = 4
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, which, 1e-8, block_size=2, min_bas=12, max_bas=24,
+                                                    max_iters=400)
+    assert nconv >= k, (nconv, niter, res)
+    check_found(ev[:k], w, which)
+    check_schur(be64, A, Q, R, k, 1e-8)
+    for o in (Q, R, A):
+        o.delete()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_jada_spin_chain_L14(be64, fp32):
+    """the reference's integration case (drivers/integration_tests.cmake:9-10:
+    subspacejada on spinSZ14 to 1e-8), smallest eigenvalues vs dense"""
+    L = 14
+    A = be64.spinchain_sz(L, L // 2)
+    dense = ol.rows_to_dense(ol.spin_sz_rows(L, L // 2, 0), A.nrows)
+    w = np.linalg.eigvalsh(dense)
+    k = 4
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, "SR", 1e-8, block_size=4, min_bas=16, max_bas=32,
+                                                    max_iters=200, inner_max_iters=10, inner_fp32=fp32)
+    assert nconv >= k, (nconv, niter, res)
+    check_found(ev[:k], w, "SR")
+    check_schur(be64, A, Q, R, k, 1e-8)
+    for o in (Q, R, A):
+        o.delete()
+
+
+def test_jada_rejects_unsupported(be64):
+    import ctypes
+    import phist_amd as pa
+    A = be64.laplace3d(4, 4, 4)
+    opts = pa.JadaOpts()
+    be64.L.phist_jadaOpts_setDefaults(ctypes.byref(opts))
+    assert opts.numEigs == 6 and opts.maxBas == 20 and abs(opts.convTol - 1e-12) < 1e-20  # reference defaults
+    op = pa.LinearOp()
+    fl = ctypes.c_int(0)
+    be64.L.phist_DlinearOp_wrap_sparseMat(ctypes.byref(op), A.h, ctypes.byref(fl))
+    Q, R = be64.mvec(A.domain_map, 6), be64.sdmat(6, 6)
+    ev, res = np.zeros(6), np.zeros(6)
+    nc, ni = ctypes.c_int(0), ctypes.c_int(0)
+    # symmetry GENERAL (the default) is not implemented: -99, no work done
+    be64.L.phist_Dsubspacejada(ctypes.byref(op), None, opts, Q.h, R.h, ev.ctypes.data_as(ctypes.c_void_p),
+                               res.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nc), ctypes.byref(ni),
+                               ctypes.byref(fl))
+    assert fl.value == -99
+    for o in (Q, R, A):
+        o.delete()
