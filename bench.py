@@ -227,6 +227,272 @@ def jada_bench(be, args):
     A.delete()
 
 
+def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
+    """Batched shifted MINRES: honeycomb sheet of W x L = 8,000,000 sites (nearest neighbours -1, on-site
+    U(-1/2, 1/2)), 32 shifts uniform in [-2.5, 2.5], 32 unit right-hand sides (e_i at spread sites) in one
+    block, tol 1e-6 relative, FP64 and FP32 iterations."""
+    t0 = time.perf_counter()
+    A = be.honeycomb(W, L, 0, 0, 0, 1.0, 77)
+    N = A.nrows
+    print(f"bench: honeycomb {W}x{L} ({N} rows, {A.nnz} nnz) built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    shifts = np.random.default_rng(11).uniform(-2.5, 2.5, nsys)
+    sites = (np.arange(nsys) * (N // nsys) + N // (2 * nsys)).astype(np.int64)
+    m = A.domain_map
+    B = be.mvec(m, nsys)
+    Bfull = np.zeros((N, nsys))  # b_j = e_{sites[j]}, set through the host mirror
+    Bfull[sites, np.arange(nsys)] = 1.0
+    B.set(Bfull)
+    del Bfull
+    results = {}
+    for label, fp32 in (("fp64", False), ("fp32", True)):
+        X = be.mvec(m, nsys)
+        if fp32:  # cached FP32 copy of A: built outside the timed solve
+            b1, x1 = be.mvec(m, 1), be.mvec(m, 1)
+            be.minres_shifted(A, [0.0], b1, x1, 1.0, 1, fp32=True)
+            b1.delete()
+            x1.delete()
+        be.synchronize()
+        t0 = time.perf_counter()
+        it, rr, unconv = be.minres_shifted(A, shifts, B, X, tol, max_iter, fp32=fp32)
+        be.synchronize()
+        dt = time.perf_counter() - t0
+        iters = int(it.max())
+        results[label] = {"seconds": round(dt, 3), "max_iters": iters, "median_iters": int(np.median(it)),
+                          "unconverged": int(unconv), "max_relres": float(rr.max()),
+                          "ms_per_iteration": round(1e3 * dt / max(1, iters), 4)}
+        print(f"bench: minres {label}: {dt:.2f} s, max {iters} iterations, {unconv} unconverged",
+              file=sys.stderr, flush=True)
+        X.delete()
+    out = {"metric": "batched shifted MINRES time (32 shifts x 32 RHS, tol 1e-6)", "value": results["fp32"]["seconds"],
+           "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 and f64",
+           "data": "synthetic (honeycomb sheet with seeded on-site disorder, unit right-hand sides)",
+           "config": {"workload": f"honeycomb {W}x{L} ({N} rows), nearest neighbours, on-site U(-0.5,0.5), "
+                                  f"{nsys} shifts in [-2.5,2.5], unit RHS, tol {tol}, maxIter {max_iter}"},
+           "fp64": results["fp64"], "fp32": results["fp32"]}
+    print(json.dumps(out), flush=True)
+    B.delete()
+    A.delete()
+
+
+def carp_bytes(nnz, n, nvec, s=8):
+    """bytes of one forward + backward Kaczmarz sweep: every row is visited twice and reads its index
+    (4), row pointer (8), entries (s+4 each) and, per column of X, reads and writes x at each entry"""
+    return 2 * (nnz * (s + 4) + n * 12 + 2 * nnz * nvec * s)
+
+
+def carp_bench(be, args, W=2048, L=2048):
+    """CARP sweeps X <- DKSWP(A - sigma I, X) as in examples/kernels/bench_carp.cpp (sigma 42.9, omega 1.5,
+    rhs NULL) on a W x L graphene sheet (long-range hopping, on-site disorder 0.2, the reference driver's
+    graphene<L> problem), nvec columns.  cpu_baseline: the oracle's sequential sweep on 1/64 of the rows."""
+    import oracle_lib as ol
+    nvec = args.nvec
+    t0 = time.perf_counter()
+    A = be.honeycomb(W, L, 0, 0, 1, 0.2, 7)
+    N = A.nrows
+    print(f"bench: graphene {W}x{L} ({N} rows, {A.nnz} nnz) built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    m = A.domain_map
+    X = be.mvec(m, nvec)
+    X.random()
+    sig = np.full(nvec, 42.9)
+    om = np.full(nvec, 1.5)
+    t0 = time.perf_counter()
+    W_ = be.carp_setup(A, sig)
+    cp, _ = W_.colors()
+    ncol = len(cp) - 1
+    print(f"bench: CARP setup (distance-2 colouring: {ncol} colours) {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    iflag = ctypes.c_int(0)
+    for _ in range(args.warmup):
+        W_.sweep(sig, None, X, om)
+    be.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        W_.sweep(sig, None, X, om)
+    be.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    be.L.phist_amd_kernel_timer(1, ctypes.byref(iflag))
+    for _ in range(3):
+        W_.sweep(sig, None, X, om)
+    kms, kcount = ctypes.c_double(0.0), ctypes.c_int64(0)
+    be.L.phist_amd_kernel_timer_read(ctypes.byref(kms), ctypes.byref(kcount), ctypes.byref(iflag))
+    be.L.phist_amd_kernel_timer(0, ctypes.byref(iflag))
+    dev_ms = kms.value / max(1, kcount.value)
+    nbytes = carp_bytes(A.nnz, N, nvec)
+    cpu = None
+    if not args.no_cpu:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        ro, ci, va = A.crs()
+        ns = N // 64
+        nnz_s = int(ro[ns] - 1)
+        ro_s, ci_s, va_s = ro[:ns + 1].copy(), ci[:nnz_s].copy(), va[:nnz_s].copy()
+        del ro, ci, va
+        keep = ci_s <= ns  # the sample is the leading ns x ns block
+        cnt = np.zeros(ns + 1, np.int64)
+        rows_of = np.repeat(np.arange(ns), np.diff(ro_s))
+        np.add.at(cnt, rows_of[keep] + 1, 1)
+        ro_s = np.cumsum(cnt) + 1
+        ci_s, va_s = ci_s[keep], va_s[keep]
+        xs = np.random.default_rng(3).uniform(-1, 1, (ns, nvec))
+        seq = np.concatenate([np.arange(ns), np.arange(ns)[::-1]]).astype(np.int32)
+        times = []
+        tb = time.perf_counter()
+        while time.perf_counter() - tb < 10.0 or len(times) < 3:
+            t1 = time.perf_counter()
+            ol.kacz_rows(ro_s, ci_s, va_s, sig, om, xs, seq)
+            times.append(time.perf_counter() - t1)
+            if len(times) >= 100:
+                break
+        tc = float(np.median(times))
+        cpu = {"value": round(carp_bytes(len(ci_s), ns, nvec) / tc / 1e9, 3), "unit": "GB/s", "cores": 1,
+               "kind": "port", "ms_per_step": round(tc * 1e3, 3),
+               "sample": f"leading {ns} x {ns} block ({len(ci_s)} nnz), lexicographic forward + backward "
+                         f"sweep, nvec={nvec}, median of {len(times)} runs"}
+    out = {"metric": f"CARP sweep GB/s (graphene {W}x{L}, nvec={nvec}, FP64)",
+           "value": round(nbytes / (ms * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (graphene sheet with seeded on-site disorder, KISS random X)",
+           "config": {"workload": f"carp_sweep (forward + backward, {ncol} colours), graphene {W}x{L} "
+                                  f"long-range, disorder 0.2, sigma 42.9, omega 1.5, rhs NULL",
+                      "rows": N, "nnz": A.nnz, "nvec": nvec, "colours": ncol, "bytes_per_sweep": nbytes,
+                      "device_ms_per_sweep": round(dev_ms, 4)},
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / (dev_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(nbytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": None},
+           "cpu_baseline": cpu}
+    W_.delete()
+    X.delete()
+    A.delete()
+    print(json.dumps(out), flush=True)
+
+
+def serve_bench(be, args):
+    """Served workload: Poisson arrivals (rate --rate per second for --duration seconds) of
+    shifted solves (1-16 right-hand sides, tol 1e-6, FP32 iterations, shifts keeping A + sI
+    positive definite) and eigenvalue requests (4-8 smallest pairs, tol 1e-8, FP32 correction
+    equations), 85 % / 15 %, over 3D Laplacians 32^3..128^3 and XXZ chains L=16..22, through the
+    library's continuous-batching loop (32 slots per MINRES batch, 200 GB work-space budget).
+    Latency = submission to completion; all buffers are allocated before the timed run."""
+    rng = np.random.default_rng(2024)
+    mats = []
+    t0 = time.perf_counter()
+    for n in (32, 64, 96, 128):
+        mats.append(("lap%d" % n, be.laplace3d(n, n, n), 0.0))  # spectrum (0, 12)
+    for L in (16, 18, 20, 22):
+        mats.append(("xxz%d" % L, be.spinchain_sz(L, L // 2), 1.8 * L))  # spectrum above about -1.78 L
+    print(f"bench: {len(mats)} matrices built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    srv = be.server(budget_bytes=200 * 2**30, slots=32)
+    # pre-allocated buffers: per matrix a ring of right-hand-side / solution blocks
+    ring = 24
+    pools = []
+    for name, A, lo in mats:
+        Bh = rng.uniform(-1, 1, (A.nrows, 16))
+        pools.append([[be.mvec(A.domain_map, 16, Bh), be.mvec(A.domain_map, 16), False] for _ in range(ring)])
+        print(f"bench: buffers for {name} ({A.nrows} rows) ready", file=sys.stderr, flush=True)
+    for name, A, lo in mats:  # FP32 copies of the matrices, made before the timed run
+        b1, x1 = be.mvec(A.domain_map, 1), be.mvec(A.domain_map, 1)
+        be.minres_shifted(A, [0.0], b1, x1, 1.0, 1, fp32=True)
+        b1.delete()
+        x1.delete()
+    print("bench: FP32 matrix copies ready; serving", file=sys.stderr, flush=True)
+    nreq = rng.poisson(args.rate * args.duration)
+    arrivals = np.sort(rng.uniform(0.0, args.duration, nreq))
+    kinds = rng.uniform(size=nreq) < 0.15
+    which_mat = rng.integers(0, len(mats), nreq)
+    be.synchronize()
+    live = {}  # id -> (kind, mat index, pool slot or JD objects, t_arrival)
+    lat = {"solve": [], "jada": []}
+    systems = 0
+    unconv = 0
+    waiting = []  # arrivals whose client buffers are all busy
+    T0 = time.perf_counter()
+    i = 0
+    last_print = T0
+    while i < nreq or live or waiting:
+        now = time.perf_counter() - T0
+        while i < nreq and arrivals[i] <= now:
+            waiting.append(i)
+            i += 1
+        still = []
+        for r in waiting:
+            mi = int(which_mat[r])
+            name, A, lo = mats[mi]
+            if kinds[r]:
+                if A.nrows > 300000:  # eigenvalue requests on the smaller matrices
+                    mi = int(rng.integers(0, 2)) * 4
+                    name, A, lo = mats[mi]
+                k = int(rng.integers(4, 9))
+                rid, Q, R, ev, res = srv.submit_jada(A, k, "SR", 1e-8, block_size=4, min_bas=k + 12,
+                                                     max_bas=k + 32, max_iters=400, inner_max_iters=10, fp32=True)
+                live[rid] = ("jada", mi, (Q, R), arrivals[r])
+                continue
+            slot = next((j for j, p in enumerate(pools[mi]) if not p[2]), None)
+            if slot is None:
+                still.append(r)
+                continue
+            nrhs = int(rng.integers(1, 17))
+            shifts = lo + rng.uniform(0.5, 3.0, nrhs)
+            B, X, _ = pools[mi][slot]
+            pools[mi][slot][2] = True
+            rid = srv.submit_solve(A, shifts, B, X, 1e-6, 5000, fp32=True)
+            live[rid] = ("solve", mi, slot, arrivals[r], nrhs)
+        waiting = still
+        ts = time.perf_counter()
+        srv.step()
+        if time.perf_counter() - ts > 2.0:
+            print(f"bench: serve step took {time.perf_counter() - ts:.1f} s (live {len(live)})", file=sys.stderr,
+                  flush=True)
+        done = []
+        for rid, info in live.items():
+            st, latency, it = srv.status(rid)
+            if st >= 2 or st < 0:
+                done.append(rid)
+                t_complete = time.perf_counter() - T0
+                lat[info[0]].append(t_complete - info[3])  # from arrival (includes client-side wait)
+                if info[0] == "solve":
+                    systems += info[4]
+                    pools[info[1]][info[2]][2] = False
+                else:
+                    info[2][0].delete()
+                    info[2][1].delete()
+                if st != 2:
+                    unconv += 1
+        for rid in done:
+            srv.release(rid)
+            del live[rid]
+        if time.perf_counter() - last_print > 2.0:
+            last_print = time.perf_counter()
+            print(f"bench: serve t={now:.1f}s submitted {i}/{nreq} live {len(live)} waiting {len(waiting)}",
+                  file=sys.stderr, flush=True)
+    wall = time.perf_counter() - T0
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)) * 1e3, 2) if len(a) else None
+
+    alll = lat["solve"] + lat["jada"]
+    out = {"metric": "served p50/p99 latency and solves/s", "value": pct(alll, 99), "unit": "ms (p99)",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f32 solves / f64 outer JD",
+           "data": "synthetic (generated matrices, random right-hand sides, Poisson arrivals)",
+           "config": {"workload": "Poisson arrivals of shifted solves (1-16 rhs, tol 1e-6) and JD requests "
+                                  "(4-8 pairs, tol 1e-8) on 3D Laplacians 32^3..128^3 and XXZ L=16..22",
+                      "rate_per_s": args.rate, "duration_s": args.duration, "requests": nreq,
+                      "slots": 32, "budget_bytes": 200 * 2**30},
+           "p50_ms": pct(alll, 50), "p99_ms": pct(alll, 99),
+           "solve": {"count": len(lat["solve"]), "p50_ms": pct(lat["solve"], 50), "p99_ms": pct(lat["solve"], 99)},
+           "jada": {"count": len(lat["jada"]), "p50_ms": pct(lat["jada"], 50), "p99_ms": pct(lat["jada"], 99)},
+           "solves_per_s": round(systems / wall, 2), "wall_s": round(wall, 2), "unconverged": unconv}
+    print(json.dumps(out), flush=True)
+    for pool in pools:
+        for B, X, _ in pool:
+            B.delete()
+            X.delete()
+    srv.delete()
+    for _, A, _ in mats:
+        A.delete()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,11 +501,14 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada"], default="spmv",
+    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada", "minres", "serve", "carp"],
+                    default="spmv",
                     help="spmv (default): the fused SpMV line; gram / cholqr / mvsd (fused_mvsd_mvTmv, "
                          "W = V*C and W'W, beta=0): the tall-skinny configuration (16,777,216 x k, U(-1,1)) "
                          "reported in the same format")
     ap.add_argument("--k", type=int, default=8, help="block width for --op gram/cholqr")
+    ap.add_argument("--rate", type=float, default=40.0, help="--op serve: mean request arrivals per second")
+    ap.add_argument("--duration", type=float, default=20.0, help="--op serve: seconds of arrivals")
     ap.add_argument("--problem", default="xxz24",
                     help="--op jada: xxz<L> (S_z=0 spin chain, 16 smallest) or lap<n> (n^3 Laplacian, 8 smallest)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -258,6 +527,12 @@ def main():
     iflag = ctypes.c_int(0)
     if args.op == "jada":
         return jada_bench(be, args)
+    if args.op == "minres":
+        return minres_bench(be, args)
+    if args.op == "serve":
+        return serve_bench(be, args)
+    if args.op == "carp":
+        return carp_bench(be, args)
     if args.op != "spmv":
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))

@@ -179,9 +179,12 @@ void SUBR_(fused_mvsdi_mvTmv)(ST_ alpha, T_(const_mvec_ptr) V, T_(mvec_ptr) W, T
 void SUBR_(fused_mvsd_mvTmv)(ST_ alpha, T_(const_mvec_ptr) V, T_(const_sdMat_ptr) C, ST_ beta, T_(mvec_ptr) W,
                              T_(sdMat_ptr) WtW, int* iflag);
 
-/* phist_kernels_carp_decl.h:35-102 -- CARP sweeps (multicoloured Kaczmarz).
- * Not provided yet: every entry returns -99, the reference's behaviour for
- * kernel libraries without CARP (src/kernels/common/kernels_no_carp.cpp). */
+/* phist_kernels_carp_decl.h:35-102 -- CARP sweeps (Kaczmarz), run on the
+ * device by multicolouring: rows of one colour share no entry of x and update
+ * in parallel; a sweep runs the colours forward, then backward.  Column j of
+ * X uses shift sigma[j] and relaxation omega[j]; Rhs may be NULL (b = 0).
+ * The augmented variants (_aug, _aug_rc) return -99, the reference's answer
+ * of kernel libraries without them (src/kernels/common/kernels_no_carp.cpp). */
 void SUBR_(carp_setup)(T_(const_sparseMat_ptr) A, int numShifts, ST_ const sigma[], void** work, int* iflag);
 void SUBR_(carp_sweep)(T_(const_sparseMat_ptr) A, ST_ const sigma[], T_(const_mvec_ptr) Rhs, T_(mvec_ptr) X,
                        void* const work, MT_ const* omega, int* iflag);

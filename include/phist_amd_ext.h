@@ -113,6 +113,14 @@ void phist_amd_honeycomb_dims(const phist_amd_honeycomb_params* p, phist_gidx* n
 int phist_amd_honeycomb_rowfunc(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
 int phist_amd_honeycomb_rowfunc_f(phist_gidx row, phist_lidx* nnz, phist_gidx* cols, void* vals, void* arg);
 
+/* colour classes of a CARP work object (from phist_[DS]carp_setup or _rc):
+   ncolors colours; rows rows[color_ptr[c] .. color_ptr[c+1]) have colour c,
+   ascending.  color_ptr needs ncolors+1 entries, rows nrows; either may be
+   NULL (query ncolors first).  A sweep visits colours 0..ncolors-1, then
+   ncolors-1..0. */
+void phist_amd_Dcarp_colors(void* work, int* ncolors, int64_t* color_ptr, int32_t* rows, int* iflag);
+void phist_amd_Scarp_colors(void* work, int* ncolors, int64_t* color_ptr, int32_t* rows, int* iflag);
+
 #ifdef __cplusplus
 }
 #endif

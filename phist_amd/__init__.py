@@ -296,6 +296,13 @@ class Backend:
             raise PhistError("blockedMINRES_shifted", fl.value)
         return it, rr, fl.value
 
+    def server(self, budget_bytes=200 * 2**30, slots=16):
+        return Server(self, budget_bytes, slots)
+
+    def carp_setup(self, A, sigma_r, sigma_i=None):
+        """phist_XcarpSetup(_rc): a work object for Kaczmarz sweeps over A - sigma_j I"""
+        return CarpWork(self, A, sigma_r, sigma_i)
+
     # -- kernels (thin) ------------------------------------------------------
     def spmv(self, alpha, A, x, beta, y):
         self.call("sparseMat_times_mvec", self.s(alpha), A.h, x.h, self.s(beta), y.h)
@@ -466,3 +473,125 @@ class _BorrowedMap:
 
     def __init__(self, n, h):
         self.n, self.h = n, h
+
+
+class CarpWork:
+    """work object of phist_XcarpSetup / _rc; sweep() = phist_XcarpSweep / _rc
+    (kernels in phist_kernels_carp_decl.h:35-102)"""
+
+    def __init__(self, be, A, sigma_r, sigma_i=None):
+        self.be, self.A = be, A
+        sr = np.ascontiguousarray(sigma_r, be.dtype)
+        self.h = ctypes.c_void_p()
+        if sigma_i is None:
+            be.call("carp_setup", A.h, ctypes.c_int(len(sr)), _ptr(sr), ctypes.byref(self.h))
+        else:
+            si = np.ascontiguousarray(sigma_i, be.dtype)
+            be.call("carp_setup_rc", A.h, ctypes.c_int(len(sr)), _ptr(sr), _ptr(si), ctypes.byref(self.h))
+
+    def sweep(self, sigma_r, B, X, omega, sigma_i=None, Xi=None):
+        """forward + backward sweep in place on X (and Xi for complex shifts); B None means b = 0"""
+        be = self.be
+        sr = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma_r, be.dtype), (X.nvec,)))
+        om = np.ascontiguousarray(np.broadcast_to(np.asarray(omega, be.dtype), (X.nvec,)))
+        bh = B.h if B is not None else None
+        if Xi is None:
+            be.call("carp_sweep", self.A.h, _ptr(sr), bh, X.h, self.h, _ptr(om))
+        else:
+            si = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma_i, be.dtype), (X.nvec,)))
+            be.call("carp_sweep_rc", self.A.h, _ptr(sr), _ptr(si), bh, X.h, Xi.h, self.h, _ptr(om))
+
+    def colors(self):
+        """(color_ptr, rows): rows[color_ptr[c]:color_ptr[c+1]] have colour c"""
+        f = getattr(self.be.L, f"phist_amd_{self.be.p}carp_colors")
+        nc, fl = ctypes.c_int(0), ctypes.c_int(0)
+        f(self.h, ctypes.byref(nc), None, None, ctypes.byref(fl))
+        if fl.value:
+            raise PhistError("carp_colors", fl.value)
+        cp = np.zeros(nc.value + 1, np.int64)
+        rows = np.zeros(max(1, self.A.nrows), np.int32)
+        f(self.h, ctypes.byref(nc), _ptr(cp), _ptr(rows), ctypes.byref(fl))
+        if fl.value:
+            raise PhistError("carp_colors", fl.value)
+        return cp, rows[:self.A.nrows]
+
+    def delete(self):
+        if self.h:
+            self.be.call("carp_destroy", self.A.h, self.h)
+            self.h = None
+
+
+class Server:
+    """phist_amd_server_*: continuous-batching loop for shifted solves and JD requests (FP64 interface)"""
+    QUEUED, RUNNING, DONE, DONE_UNCONVERGED = 0, 1, 2, 3
+
+    def __init__(self, be, budget_bytes, slots):
+        assert be.p == "D"
+        self.be, self.L = be, be.L
+        self.h = ctypes.c_void_p()
+        fl = ctypes.c_int(0)
+        self.L.phist_amd_server_create(ctypes.byref(self.h), ctypes.c_int64(int(budget_bytes)), ctypes.c_int(slots),
+                                       ctypes.byref(fl))
+        if fl.value:
+            raise PhistError("server_create", fl.value)
+        self._keep = {}
+
+    def _chk(self, name, fl):
+        if fl.value < 0:
+            raise PhistError(name, fl.value)
+
+    def submit_solve(self, A, shifts, B, X, tol, max_iter, fp32=False):
+        sh = np.ascontiguousarray(shifts, np.float64)
+        rid, fl = ctypes.c_int64(0), ctypes.c_int(0)
+        self.L.phist_amd_server_submit_solve(self.h, A.h, ctypes.c_int(len(sh)), _ptr(sh), B.h, X.h,
+                                             ctypes.c_double(tol), ctypes.c_int(max_iter), ctypes.c_int(int(fp32)),
+                                             ctypes.byref(rid), ctypes.byref(fl))
+        self._chk("server_submit_solve", fl)
+        self._keep[rid.value] = (sh,)
+        return rid.value
+
+    def submit_jada(self, A, num_eigs, which="SR", tol=1e-8, block_size=4, min_bas=None, max_bas=None,
+                    max_iters=300, inner_max_iters=10, fp32=False):
+        """returns (id, Q, R, ev, resNorm); Q/R are Mvec/SdMat owned by the caller"""
+        be = self.be
+        opts = JadaOpts()
+        self.L.phist_jadaOpts_setDefaults(ctypes.byref(opts))
+        opts.numEigs, opts.which, opts.convTol = num_eigs, EIGSORT[which], tol
+        opts.symmetry, opts.blockSize = HERMITIAN, block_size
+        opts.minBas = min_bas if min_bas is not None else max(10, num_eigs + block_size)
+        opts.maxBas = max_bas if max_bas is not None else opts.minBas + 2 * block_size + 10
+        opts.maxIters, opts.innerSolvType, opts.innerSolvMaxIters = max_iters, LINSOLV["MINRES"], inner_max_iters
+        nq = num_eigs + block_size - 1
+        Q, R = be.mvec(A.domain_map, nq), be.sdmat(nq, nq)
+        ev, res = np.zeros(nq), np.zeros(num_eigs)
+        rid, fl = ctypes.c_int64(0), ctypes.c_int(0)
+        self.L.phist_amd_server_submit_jada(self.h, A.h, ctypes.byref(opts), ctypes.c_int(int(fp32)), Q.h, R.h,
+                                            _ptr(ev), _ptr(res), ctypes.byref(rid), ctypes.byref(fl))
+        self._chk("server_submit_jada", fl)
+        self._keep[rid.value] = (opts, ev, res)
+        return rid.value, Q, R, ev, res
+
+    def step(self):
+        pending, fl = ctypes.c_int(0), ctypes.c_int(0)
+        self.L.phist_amd_server_step(self.h, ctypes.byref(pending), ctypes.byref(fl))
+        self._chk("server_step", fl)
+        return pending.value
+
+    def status(self, rid):
+        st, lat, it, fl = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_int(0), ctypes.c_int(0)
+        self.L.phist_amd_server_status(self.h, ctypes.c_int64(rid), ctypes.byref(st), ctypes.byref(lat),
+                                       ctypes.byref(it), ctypes.byref(fl))
+        self._chk("server_status", fl)
+        return st.value, lat.value, it.value
+
+    def release(self, rid):
+        fl = ctypes.c_int(0)
+        self.L.phist_amd_server_release(self.h, ctypes.c_int64(rid), ctypes.byref(fl))
+        self._chk("server_release", fl)
+        self._keep.pop(rid, None)
+
+    def delete(self):
+        if self.h:
+            fl = ctypes.c_int(0)
+            self.L.phist_amd_server_delete(self.h, ctypes.byref(fl))
+            self.h = None

@@ -255,6 +255,14 @@ template <typename T> void blas_elemwise(T alpha, const Mvec<T>* V, Mvec<T>* W);
 template <typename T> void blas_dot(const Mvec<T>* V, const Mvec<T>* W, T* out_host);
 template <typename T> void blas_copy(const Mvec<T>* X, Mvec<T>* Y);
 
+// CARP: multicoloured Kaczmarz sweeps (carp.hip)
+template <typename T> void* carp_setup(const SparseMat<T>* A, int nshifts);
+template <typename T> void carp_destroy(void* work);
+template <typename T>
+void carp_sweep(const SparseMat<T>* A, void* work, const T* sr, const T* si, const Mvec<T>* B, Mvec<T>* Xr,
+                Mvec<T>* Xi, const T* omega);
+template <typename T> void carp_colors(void* work, int* ncolors, int64_t* color_ptr, int32_t* rows);
+
 // tall-skinny GEMMs; the sdMat results are written to the device copy and
 // copied to the host copy before returning
 template <typename T> void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C);

@@ -28,3 +28,17 @@ inline bool spmv_trivial(const SparseMat<T>* A, T alpha) {
 #undef ST
 #undef P
 #undef H
+
+// colour classes of a CARP work object made by phist_[DS]carp_setup(_rc)
+extern "C" void phist_amd_Dcarp_colors(void* work, int* ncolors, int64_t* color_ptr, int32_t* rows, int* iflag) {
+  guard(iflag, "phist_amd_Dcarp_colors", [&] {
+    carp_colors<double>(work, ncolors, color_ptr, rows);
+    *iflag = 0;
+  });
+}
+extern "C" void phist_amd_Scarp_colors(void* work, int* ncolors, int64_t* color_ptr, int32_t* rows, int* iflag) {
+  guard(iflag, "phist_amd_Scarp_colors", [&] {
+    carp_colors<float>(work, ncolors, color_ptr, rows);
+    *iflag = 0;
+  });
+}

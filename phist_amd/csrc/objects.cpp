@@ -109,23 +109,64 @@ void mvec_set_block(Mvec<T>* V, const Mvec<T>* B, int jmin, int jmax) {
   blas_copy<T>(B, &dst);
 }
 
-// 2D copies between the device block and the host mirror (same layout)
+// Row-block copies between host memory (row pitch hp bytes) and device
+// memory (row pitch dp bytes), `rows` rows of `width` bytes.  Contiguous
+// blocks go as one linear copy; a strided side is packed on its own memory
+// (host loop, or a device-to-device 2D copy through a staging buffer) so the
+// host <-> device transfer is always linear: a 2D copy from host memory with
+// millions of short rows is far slower than the link.
+void copy_rows_h2d(void* dev, size_t dp, const void* host, size_t hp, size_t width, size_t rows) {
+  hipStream_t st = rt().stream;
+  const char* src = static_cast<const char*>(host);
+  std::vector<char> pack;
+  if (hp != width) {
+    pack.resize(width * rows);
+    for (size_t r = 0; r < rows; r++) memcpy(pack.data() + r * width, src + r * hp, width);
+    src = pack.data();
+  }
+  if (dp == width) {
+    PHX_HIP(hipMemcpyAsync(dev, src, width * rows, hipMemcpyHostToDevice, st));
+  } else {
+    void* stage = device_work(width * rows);
+    PHX_HIP(hipMemcpyAsync(stage, src, width * rows, hipMemcpyHostToDevice, st));
+    PHX_HIP(hipMemcpy2DAsync(dev, dp, stage, width, width, rows, hipMemcpyDeviceToDevice, st));
+  }
+  sync_stream();
+}
+
+void copy_rows_d2h(void* host, size_t hp, const void* dev, size_t dp, size_t width, size_t rows) {
+  hipStream_t st = rt().stream;
+  const void* src = dev;
+  if (dp != width) {
+    void* stage = device_work(width * rows);
+    PHX_HIP(hipMemcpy2DAsync(stage, width, dev, dp, width, rows, hipMemcpyDeviceToDevice, st));
+    src = stage;
+  }
+  if (hp == width) {
+    PHX_HIP(hipMemcpyAsync(host, src, width * rows, hipMemcpyDeviceToHost, st));
+    sync_stream();
+  } else {
+    std::vector<char> pack(width * rows);
+    PHX_HIP(hipMemcpyAsync(pack.data(), src, width * rows, hipMemcpyDeviceToHost, st));
+    sync_stream();
+    char* dst = static_cast<char*>(host);
+    for (size_t r = 0; r < rows; r++) memcpy(dst + r * hp, pack.data() + r * width, width);
+  }
+}
+
+// copies between the device block and the host mirror (same layout)
 template <typename T>
 void mvec_to_device(Mvec<T>* V) {
   if (V->nrows == 0) return;
   T* h = V->host();
-  PHX_HIP(hipMemcpy2DAsync(V->dev(), V->lda * sizeof(T), h, V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows,
-                           hipMemcpyHostToDevice, rt().stream));
-  sync_stream();
+  copy_rows_h2d(V->dev(), V->lda * sizeof(T), h, V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows);
 }
 
 template <typename T>
 void mvec_from_device(Mvec<T>* V) {
   if (V->nrows == 0) return;
   T* h = V->host();
-  PHX_HIP(hipMemcpy2DAsync(h, V->lda * sizeof(T), V->dev(), V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows,
-                           hipMemcpyDeviceToHost, rt().stream));
-  sync_stream();
+  copy_rows_d2h(h, V->lda * sizeof(T), V->dev(), V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows);
 }
 
 template <typename T>
@@ -134,14 +175,10 @@ void mvec_get_data(const Mvec<T>* V, T* data, int64_t lda, int row_major) {
   PHX_REQUIRE(lda >= (row_major ? V->nvec : V->nrows), PHIST_INVALID_INPUT, "lda too small");
   if (V->nrows == 0) return;
   if (row_major) {
-    PHX_HIP(hipMemcpy2DAsync(data, lda * sizeof(T), V->dev(), V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows,
-                             hipMemcpyDeviceToHost, rt().stream));
-    sync_stream();
+    copy_rows_d2h(data, lda * sizeof(T), V->dev(), V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows);
   } else {
     std::vector<T> tmp((size_t)V->nrows * V->nvec);
-    PHX_HIP(hipMemcpy2DAsync(tmp.data(), V->nvec * sizeof(T), V->dev(), V->lda * sizeof(T), V->nvec * sizeof(T),
-                             V->nrows, hipMemcpyDeviceToHost, rt().stream));
-    sync_stream();
+    copy_rows_d2h(tmp.data(), V->nvec * sizeof(T), V->dev(), V->lda * sizeof(T), V->nvec * sizeof(T), V->nrows);
     for (int64_t i = 0; i < V->nrows; i++)
       for (int j = 0; j < V->nvec; j++) data[(int64_t)j * lda + i] = tmp[(size_t)i * V->nvec + j];
   }
@@ -153,16 +190,12 @@ void mvec_set_data(Mvec<T>* V, const T* data, int64_t lda, int row_major) {
   PHX_REQUIRE(lda >= (row_major ? V->nvec : V->nrows), PHIST_INVALID_INPUT, "lda too small");
   if (V->nrows == 0) return;
   if (row_major) {
-    PHX_HIP(hipMemcpy2DAsync(V->dev(), V->lda * sizeof(T), data, lda * sizeof(T), V->nvec * sizeof(T), V->nrows,
-                             hipMemcpyHostToDevice, rt().stream));
-    sync_stream();
+    copy_rows_h2d(V->dev(), V->lda * sizeof(T), data, lda * sizeof(T), V->nvec * sizeof(T), V->nrows);
   } else {
     std::vector<T> tmp((size_t)V->nrows * V->nvec);
     for (int64_t i = 0; i < V->nrows; i++)
       for (int j = 0; j < V->nvec; j++) tmp[(size_t)i * V->nvec + j] = data[(int64_t)j * lda + i];
-    PHX_HIP(hipMemcpy2DAsync(V->dev(), V->lda * sizeof(T), tmp.data(), V->nvec * sizeof(T), V->nvec * sizeof(T),
-                             V->nrows, hipMemcpyHostToDevice, rt().stream));
-    sync_stream();
+    copy_rows_h2d(V->dev(), V->lda * sizeof(T), tmp.data(), V->nvec * sizeof(T), V->nvec * sizeof(T), V->nrows);
   }
 }
 

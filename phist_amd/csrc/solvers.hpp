@@ -1,0 +1,429 @@
+// solvers.hpp -- building blocks of the device solvers (jada.cpp, serve.cpp):
+// scoped block vectors and views, the operator abstraction, and the slot-based
+// blocked MINRES engine.
+#pragma once
+#include "objects.hpp"
+#include "phist_amd_jada.h"
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+namespace phx {
+
+template <typename T>
+void krylov_lanczos(int64_t nrows, int ncols, int64_t ld, const T* v, const T* vold, T* w, const T* a, const T* b,
+                    T* nrm_host);
+template <typename T>
+void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* vold, const T* w, const T* d1, T* d2,
+                          T* x, const T* delta, const T* eps, const T* inv_gamma, const T* tau, const T* inv_beta);
+template <typename TS, typename TD>
+void convert_block(const TS* x, int64_t ldx, TD* y, int64_t ldy, int64_t nrows, int ncols);
+
+// owned block vector, freed at scope exit
+template <typename T>
+struct MvBox {
+  Mvec<T>* p = nullptr;
+  MvBox() = default;
+  MvBox(const Map* map, int nvec) : p(api::mvec_create<T>(map, nvec)) {}
+  ~MvBox() {
+    try {
+      if (p) api::mvec_delete(p);
+    } catch (...) {
+    }
+  }
+  MvBox(const MvBox&) = delete;
+  MvBox& operator=(const MvBox&) = delete;
+  Mvec<T>* operator->() const { return p; }
+  Mvec<T>* get() const { return p; }
+};
+
+// column view [j0, j1] of a block vector, deleted at scope exit
+template <typename T>
+struct View {
+  Mvec<T>* p = nullptr;
+  View(const Mvec<T>* V, int j0, int j1) { api::mvec_view_block(const_cast<Mvec<T>*>(V), &p, j0, j1); }
+  ~View() {
+    try {
+      if (p) api::mvec_delete(p);
+    } catch (...) {
+    }
+  }
+  View(const View&) = delete;
+  View& operator=(const View&) = delete;
+  Mvec<T>* operator->() const { return p; }
+  Mvec<T>* get() const { return p; }
+};
+
+template <typename T>
+struct SdBox {
+  SdMat<T>* p = nullptr;
+  SdBox() = default;
+  SdBox(int m, int n) : p(api::sdMat_create<T>(std::max(1, m), std::max(1, n))) {}
+  ~SdBox() {
+    try {
+      if (p) api::sdMat_delete(p);
+    } catch (...) {
+    }
+  }
+  SdBox(const SdBox&) = delete;
+  SdBox& operator=(const SdBox&) = delete;
+  SdMat<T>* operator->() const { return p; }
+  SdMat<T>* get() const { return p; }
+};
+
+template <typename T>
+struct SdView {
+  SdMat<T>* p = nullptr;
+  SdView(SdMat<T>* M, int i0, int i1, int j0, int j1) { api::sdMat_view_block(M, &p, i0, i1, j0, j1); }
+  ~SdView() {
+    try {
+      if (p) api::sdMat_delete(p);
+    } catch (...) {
+    }
+  }
+  SdView(const SdView&) = delete;
+  SdView& operator=(const SdView&) = delete;
+  SdMat<T>* operator->() const { return p; }
+  SdMat<T>* get() const { return p; }
+};
+
+// ---------------------------------------------------------------------------
+// operators: either a sparse matrix of this library (device fast path) or a
+// user linearOp called through its function pointers
+// ---------------------------------------------------------------------------
+template <typename T> struct LinOpT;
+template <> struct LinOpT<double> {
+  using type = phist_DlinearOp;
+  using mvec = phist_Dmvec;
+  using sd = phist_DsdMat;
+  static void apply(double alpha, const void* A, const phist_Dmvec* X, double beta, phist_Dmvec* Y, int* iflag) {
+    phist_DsparseMat_times_mvec(alpha, (phist_Dconst_sparseMat_ptr)A, X, beta, Y, iflag);
+  }
+  static void applyT(double, const void*, const phist_Dmvec*, double, phist_Dmvec*, int* iflag) {
+    *iflag = PHIST_NOT_IMPLEMENTED;
+  }
+  static void apply_shifted(double alpha, const void* A, const double* sigma, const phist_Dmvec* X, double beta,
+                            phist_Dmvec* Y, int* iflag) {
+    phist_DsparseMat_times_mvec_vadd_mvec(alpha, (phist_Dconst_sparseMat_ptr)A, sigma, X, beta, Y, iflag);
+  }
+  static void fused(double alpha, const void* A, const phist_Dmvec* V, double beta, phist_Dmvec* W, phist_DsdMat* WtW,
+                    phist_DsdMat* VtW, int* iflag) {
+    phist_Dfused_spmv_mvTmv(alpha, (phist_Dconst_sparseMat_ptr)A, V, beta, W, WtW, VtW, iflag);
+  }
+  static void destroy(phist_DlinearOp*, int* iflag) { *iflag = 0; }
+};
+template <> struct LinOpT<float> {
+  using type = phist_SlinearOp;
+  using mvec = phist_Smvec;
+  using sd = phist_SsdMat;
+  static void apply(float alpha, const void* A, const phist_Smvec* X, float beta, phist_Smvec* Y, int* iflag) {
+    phist_SsparseMat_times_mvec(alpha, (phist_Sconst_sparseMat_ptr)A, X, beta, Y, iflag);
+  }
+  static void applyT(float, const void*, const phist_Smvec*, float, phist_Smvec*, int* iflag) {
+    *iflag = PHIST_NOT_IMPLEMENTED;
+  }
+  static void apply_shifted(float alpha, const void* A, const float* sigma, const phist_Smvec* X, float beta,
+                            phist_Smvec* Y, int* iflag) {
+    phist_SsparseMat_times_mvec_vadd_mvec(alpha, (phist_Sconst_sparseMat_ptr)A, sigma, X, beta, Y, iflag);
+  }
+  static void fused(float alpha, const void* A, const phist_Smvec* V, float beta, phist_Smvec* W, phist_SsdMat* WtW,
+                    phist_SsdMat* VtW, int* iflag) {
+    phist_Sfused_spmv_mvTmv(alpha, (phist_Sconst_sparseMat_ptr)A, V, beta, W, WtW, VtW, iflag);
+  }
+  static void destroy(phist_SlinearOp*, int* iflag) { *iflag = 0; }
+};
+
+// the sparse matrix behind an operator made by linearOp_wrap_sparseMat
+template <typename T>
+const SparseMat<T>* own_matrix(const typename LinOpT<T>::type* op) {
+  if (op && op->apply == &LinOpT<T>::apply && op->apply_shifted == &LinOpT<T>::apply_shifted)
+    return reinterpret_cast<const SparseMat<T>*>(op->A);
+  return nullptr;
+}
+
+// Y = alpha*(A + diag(s))*X + beta*Y with optional column dots <X, Y>
+template <typename T>
+struct Operator {
+  const SparseMat<T>* A = nullptr;
+  const typename LinOpT<T>::type* user = nullptr;
+  void apply_shifted(T alpha, const T* s, const Mvec<T>* X, T beta, Mvec<T>* Y, T* xy_dots) const {
+    if (A) {
+      SpmvRequest req;
+      req.has_shift = s != nullptr;
+      req.has_beta = beta != T(0);
+      req.dot_vw = xy_dots != nullptr;
+      spmv_launch<T>(A, alpha, s, X, beta, Y, req, nullptr, xy_dots);
+      return;
+    }
+    int iflag = 0;
+    auto* x = reinterpret_cast<const typename LinOpT<T>::mvec*>(X);
+    auto* y = reinterpret_cast<typename LinOpT<T>::mvec*>(Y);
+    if (s) {
+      PHX_REQUIRE(user->apply_shifted != nullptr, PHIST_NOT_IMPLEMENTED, "operator has no apply_shifted");
+      user->apply_shifted(alpha, user->A, s, x, beta, y, &iflag);
+    } else {
+      user->apply(alpha, user->A, x, beta, y, &iflag);
+    }
+    PHX_REQUIRE(iflag == 0, iflag, "user operator returned %d", iflag);
+    if (xy_dots) blas_dot<T>(X, Y, xy_dots);
+  }
+};
+
+
+// copy a block into another of any precision (same shape)
+template <typename TS, typename TD>
+void copy_any(const Mvec<TS>* X, Mvec<TD>* Y) {
+  if constexpr (std::is_same<TS, TD>::value)
+    blas_copy<TD>(X, Y);
+  else
+    convert_block<TS, TD>(X->dev(), X->lda, Y->dev(), Y->lda, X->nrows, X->nvec);
+}
+
+// ---------------------------------------------------------------------------
+// MinresBatch: up to `cap` independent systems P(A + s_j I)P x_j = b_j
+// (P = I - QQ', or I without Q), one column (slot) each, advanced together by
+// MINRES with Paige & Saunders' short recurrences.  Per iteration: one
+// shifted SpMV whose fused <v, Av> dots give the Lanczos alphas, the
+// projection, k_lanczos (update + norms) and k_minres (direction, solution,
+// next Lanczos vector); the 2x2 rotations of the slots run on the host.  A
+// slot whose system has converged (|phibar| <= tol*|b|), broken down or
+// reached its iteration limit stops updating; its solution can be stored and
+// the slot loaded with the next system while the others keep iterating (the
+// slot swap of phist_jadaCorrectionSolver_def.hpp:428-466).
+// ---------------------------------------------------------------------------
+template <typename T>
+class MinresBatch {
+ public:
+  MinresBatch(const Operator<T>& op, const Map* map, int cap, const Mvec<T>* Q = nullptr)
+      : op_(op), map_(map), cap_(cap), Q_(Q), work_(map, 6 * cap), st_(cap) {
+    PHX_REQUIRE(cap >= 1 && cap <= 64, PHIST_INVALID_INPUT, "MINRES batch of %d systems", cap);
+    for (int b = 0; b < 6; b++) blk_[b].p = nullptr;
+    for (int b = 0; b < 6; b++) api::mvec_view_block(work_.get(), &blk_[b].p, b * cap, (b + 1) * cap - 1);
+    blas_put_value<T>(work_.get(), T(0));
+    if (Q_ && Q_->nvec > 0) C_.reset(new SdBox<T>(Q_->nvec, cap));
+  }
+  MinresBatch(const MinresBatch&) = delete;
+  MinresBatch& operator=(const MinresBatch&) = delete;
+
+  int capacity() const { return cap_; }
+  int free_slot() const {
+    for (int j = 0; j < cap_; j++)
+      if (!st_[j].used) return j;
+    return -1;
+  }
+  bool used(int j) const { return st_[j].used; }
+  bool active(int j) const { return st_[j].active; }
+  int iters(int j) const { return st_[j].iters; }
+  double relres(int j) const { return st_[j].relres; }
+  bool converged(int j) const { return st_[j].relres <= st_[j].tol; }
+  bool any_active() const {
+    for (const Slot& s : st_)
+      if (s.active) return true;
+    return false;
+  }
+  int n_used() const {
+    int n = 0;
+    for (const Slot& s : st_) n += s.used;
+    return n;
+  }
+
+  // all slots at once from the columns of B (x = 0)
+  template <typename TS>
+  void load_all(const Mvec<TS>* B, const std::vector<T>& shifts, const std::vector<double>& tol,
+                const std::vector<int>& max_iter) {
+    const int k = B->nvec;
+    PHX_REQUIRE(k == cap_, PHIST_INVALID_INPUT, "load_all needs %d columns", cap_);
+    blas_put_value<T>(work_.get(), T(0));
+    Mvec<T>* v = cur(V);
+    copy_any<TS, T>(B, v);
+    std::vector<T> nrm(k), sc(k);
+    blas_dot<T>(v, v, nrm.data());
+    for (int j = 0; j < k; j++) {
+      nrm[j] = std::sqrt(nrm[j]);
+      sc[j] = nrm[j] > T(0) ? T(1) / nrm[j] : T(0);
+      start(j, (double)nrm[j], shifts[j], tol[j], max_iter[j]);
+    }
+    blas_scale_cols<T>(v, sc.data(), false);
+  }
+
+  // slot j from column c of B (x = 0)
+  template <typename TS>
+  void load(int j, const Mvec<TS>* B, int c, T shift, double tol, int max_iter) {
+    for (int b = 0; b < 6; b++) {
+      View<T> col(blk_[b].p, j, j);
+      blas_put_value<T>(col.get(), T(0));
+    }
+    View<T> vj(cur(V), j, j);
+    View<TS> bc(B, c, c);
+    copy_any<TS, T>(bc.get(), vj.get());
+    T nrm;
+    blas_dot<T>(vj.get(), vj.get(), &nrm);
+    nrm = std::sqrt(nrm);
+    T sc = nrm > T(0) ? T(1) / nrm : T(0);
+    blas_scale_cols<T>(vj.get(), &sc, true);
+    start(j, (double)nrm, shift, tol, max_iter);
+  }
+
+  // solution of slot j into column c of X (any precision)
+  template <typename TD>
+  void store(int j, Mvec<TD>* X, int c) const {
+    View<T> xj(cur(XS), j, j);
+    View<TD> xc(X, c, c);
+    copy_any<T, TD>(xj.get(), xc.get());
+  }
+  template <typename TD>
+  void store_all(Mvec<TD>* X) const {
+    copy_any<T, TD>(cur(XS), X);
+  }
+  void release(int j) { st_[j] = Slot(); }
+
+  // one MINRES iteration of every active slot
+  void iterate() {
+    const int k = cap_;
+    std::vector<T> s(k), alpha(k), la(k), lb(k), bnext(k), dl(k), ep(k), ig(k), ta(k), ib(k);
+    for (int j = 0; j < k; j++) s[j] = st_[j].active ? st_[j].shift : T(0);
+    Mvec<T>* v = cur(V);
+    Mvec<T>* w = cur(W);
+    op_.apply_shifted(T(1), s.data(), v, T(0), w, alpha.data());
+    if (C_) {
+      gemm_gram<T>(T(1), Q_, w, T(0), C_->get());
+      gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
+    }
+    for (int j = 0; j < k; j++) {
+      la[j] = st_[j].active ? alpha[j] : T(0);
+      lb[j] = st_[j].active && st_[j].iters > 0 ? (T)st_[j].beta : T(0);
+    }
+    const int64_t ld = work_->lda;
+    krylov_lanczos<T>(v->nrows, k, ld, v->dev(), cur(VOLD)->dev(), w->dev(), la.data(), lb.data(), bnext.data());
+    for (int j = 0; j < k; j++) {
+      dl[j] = ep[j] = ig[j] = ta[j] = ib[j] = T(0);
+      Slot& q = st_[j];
+      if (!q.active) continue;
+      // column `it` of the Lanczos tridiagonal: beta_k above, alpha_k on and
+      // beta_{k+1} below the diagonal; two old rotations, one new one
+      const double bk = q.iters > 0 ? q.beta : 0.0, ak = (double)alpha[j], bn = (double)bnext[j];
+      const double eps_k = q.s2 * bk, dbar = q.c2 * bk;
+      const double delta = q.c1 * dbar + q.s1 * ak;
+      const double gbar = -q.s1 * dbar + q.c1 * ak;
+      const double gamma = std::hypot(gbar, bn);
+      if (!(gamma > 0.0) || !std::isfinite(gamma)) {  // singular or broken down: stop this system
+        q.active = false;
+        continue;
+      }
+      const double c = gbar / gamma, sn = bn / gamma;
+      const double tau = c * q.phibar;
+      q.phibar = -sn * q.phibar;
+      dl[j] = (T)delta;
+      ep[j] = (T)eps_k;
+      ig[j] = (T)(1.0 / gamma);
+      ta[j] = (T)tau;
+      ib[j] = bn > 0.0 ? (T)(1.0 / bn) : T(0);
+      q.c2 = q.c1;
+      q.s2 = q.s1;
+      q.c1 = c;
+      q.s1 = sn;
+      q.beta = bn;
+      q.iters++;
+      q.relres = std::fabs(q.phibar) / q.beta1;
+      if (q.relres <= q.tol || bn == 0.0 || q.iters >= q.max_iter) q.active = false;
+    }
+    krylov_minres_update<T>(v->nrows, k, ld, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(), cur(D2)->dev(),
+                            cur(XS)->dev(), dl.data(), ep.data(), ig.data(), ta.data(), ib.data());
+    std::swap(role_[V], role_[VOLD]);  // the next Lanczos vector was written over the old one
+    std::swap(role_[D1], role_[D2]);   // the new direction was written over d_{k-2}
+  }
+
+ private:
+  enum { V = 0, VOLD = 1, W = 2, D1 = 3, D2 = 4, XS = 5 };
+  struct Slot {
+    bool used = false, active = false;
+    double beta1 = 0, beta = 0, c1 = 1, s1 = 0, c2 = 1, s2 = 0, phibar = 0, relres = 1, tol = 0;
+    int iters = 0, max_iter = 0;
+    T shift = T(0);
+  };
+  struct Blk {
+    Mvec<T>* p = nullptr;
+    ~Blk() {
+      try {
+        if (p) api::mvec_delete(p);
+      } catch (...) {
+      }
+    }
+  };
+  Mvec<T>* cur(int r) const { return blk_[role_[r]].p; }
+  void start(int j, double beta1, T shift, double tol, int max_iter) {
+    Slot q;
+    q.used = true;
+    q.beta1 = beta1;
+    q.phibar = beta1;
+    q.shift = shift;
+    q.tol = tol;
+    q.max_iter = max_iter;
+    q.active = beta1 > 0.0 && max_iter > 0;
+    q.relres = beta1 > 0.0 ? 1.0 : 0.0;
+    st_[j] = q;
+  }
+
+  const Operator<T>& op_;
+  const Map* map_;
+  int cap_;
+  const Mvec<T>* Q_;
+  MvBox<T> work_;
+  Blk blk_[6];  // views of work_ (destroyed before work_)
+  int role_[6] = {0, 1, 2, 3, 4, 5};
+  std::vector<Slot> st_;
+  std::unique_ptr<SdBox<T>> C_;
+};
+
+template <typename T>
+struct MinresOut {
+  std::vector<int> iters;
+  std::vector<double> relres;
+  int unconverged = 0;
+};
+
+// k systems at once in precision TI (B and X in precision T, Q converted)
+template <typename T, typename TI>
+MinresOut<TI> minres_solve(const Operator<TI>& op, const std::vector<T>& s, const Mvec<T>* Q, const Mvec<T>* B,
+                           Mvec<T>* X, const std::vector<double>& tol, int maxIter) {
+  const int k = B->nvec;
+  MvBox<TI> q;
+  const Mvec<TI>* Qi = nullptr;
+  if (Q && Q->nvec > 0) {
+    if constexpr (std::is_same<T, TI>::value) {
+      Qi = Q;
+    } else {
+      q.p = api::mvec_create<TI>(Q->map, Q->nvec);
+      copy_any<T, TI>(Q, q.get());
+      Qi = q.get();
+    }
+  }
+  MinresBatch<TI> mb(op, B->map, k, Qi);
+  std::vector<TI> si(s.begin(), s.end());
+  mb.load_all(B, si, tol, std::vector<int>(k, maxIter));
+  while (mb.any_active()) mb.iterate();
+  mb.store_all(X);
+  MinresOut<TI> out;
+  out.iters.resize(k);
+  out.relres.resize(k);
+  for (int j = 0; j < k; j++) {
+    out.iters[j] = mb.iters(j);
+    out.relres[j] = mb.relres(j);
+    if (!mb.converged(j)) out.unconverged++;
+  }
+  return out;
+}
+
+// resumable block Jacobi-Davidson on a sparse matrix of this library
+// (defined in jada.cpp): start builds the start-up basis, step runs one outer
+// iteration (true when finished), finish writes the outputs
+struct JadaHandle;
+JadaHandle* jada_start(const SparseMat<double>* A, bool fp32, const phist_jadaOpts& opts, Mvec<double>* Q,
+                       SdMat<double>* R, double* ev, double* resNorm);
+bool jada_step(JadaHandle* h);
+void jada_finish(JadaHandle* h, int* nConv, int* nIter);
+void jada_free(JadaHandle* h);
+
+}  // namespace phx

@@ -33,7 +33,7 @@ _oracle = None
 def oracle():
     global _oracle
     if _oracle is None:
-        srcs = [os.path.join(ORACLE_DIR, f) for f in ("builtin_oracle.c", "matfuncs_oracle.c")]
+        srcs = [os.path.join(ORACLE_DIR, f) for f in ("builtin_oracle.c", "matfuncs_oracle.c", "carp_oracle.c")]
         if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < max(map(os.path.getmtime, srcs)):
             subprocess.check_call(["make", "-s", "-C", ORACLE_DIR, "all"])
         _oracle = ctypes.CDLL(ORACLE_SO)
@@ -240,3 +240,25 @@ def rows_to_dense(rows, n):
         for cc, vv in zip(c, v):
             A[r, cc] += vv
     return A
+
+
+def kacz_rows(ro, ci, va, sr, omega, xr, seq, b=None, si=None, xi=None):
+    """oracle_kacz_rows_{d,f}: Kaczmarz row updates over rows seq (0-based), in place on xr (and xi);
+    ro/ci 1-based builtin CRS, mvecs row-major (n, nvec)"""
+    dt = xr.dtype
+    f = oracle().oracle_kacz_rows_d if dt == np.float64 else oracle().oracle_kacz_rows_f
+    nvec = xr.shape[1]
+    ro = np.ascontiguousarray(ro, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    va = np.ascontiguousarray(va, dt)
+    sr = np.ascontiguousarray(np.broadcast_to(np.asarray(sr, dt), (nvec,)))
+    om = np.ascontiguousarray(np.broadcast_to(np.asarray(omega, dt), (nvec,)))
+    si_a = None if si is None else np.ascontiguousarray(np.broadcast_to(np.asarray(si, dt), (nvec,)))
+    seq = np.ascontiguousarray(seq, np.int32)
+    assert xr.flags.c_contiguous and (xi is None or (xi.flags.c_contiguous and xi.shape == xr.shape))
+    if b is not None:
+        b = np.ascontiguousarray(b, dt)
+        assert b.shape == xr.shape
+    f(p(ro), p(ci), p(va), ctypes.c_int(nvec), p(sr), p(si_a) if si_a is not None else None, p(om),
+      p(b) if b is not None else None, ctypes.c_int64(nvec), p(xr), p(xi) if xi is not None else None,
+      ctypes.c_int64(nvec), p(seq), ctypes.c_int64(len(seq)))

@@ -1,0 +1,357 @@
+"""GPU parity: CARP sweeps (phist_kernels_carp_decl.h:35-102) through the C
+ABI against the Kaczmarz oracle (oracle/carp_oracle.c, pinned in
+test_carp_oracle.py).
+
+The device sweep runs the colours of a distance-2 colouring forward, then
+backward; the rows of one colour share no entry of x, so the oracle applied
+to the same rows in colour order is the sequential sweep the device result
+must equal bit for bit (FP64 and FP32).  The colouring itself is checked:
+every row once, no two rows of a colour touching the same x entry.  The
+reference's own checks (CarpTest_def.hpp: Identity_yields_zero,
+operator_is_deterministic, -99 for the augmented variants) are repeated.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+MATS = ["Dsprandn25", "Dsprandn388", "Dsprandn_nodiag25", "Dsprandsym163", "DjadaTestMat512", "Dspshift25",
+        "Dspzero25", "Dspeye25"]
+
+
+def mm(name):
+    return os.path.join(ol.MATRICES, name + ".mm")
+
+
+def check_coloring(cp, rows, ro, ci, n):
+    assert cp[0] == 0 and cp[-1] == n and np.all(np.diff(cp) >= 0)
+    assert np.array_equal(np.sort(rows), np.arange(n))
+    for c in range(len(cp) - 1):
+        rs = rows[cp[c]:cp[c + 1]]
+        assert np.all(np.diff(rs) > 0)  # ascending inside a colour
+        supp = [np.union1d(ci[ro[i] - 1:ro[i + 1] - 1] - 1, [i]) for i in rs]
+        allc = np.concatenate(supp) if supp else np.zeros(0, np.int64)
+        assert len(np.unique(allc)) == len(allc), f"colour {c}: rows share an x entry"
+
+
+def colour_sequence(cp, rows):
+    nc = len(cp) - 1
+    fwd = [rows[cp[c]:cp[c + 1]] for c in range(nc)]
+    return np.concatenate(fwd + fwd[::-1]) if nc else np.zeros(0, np.int32)
+
+
+def run_pair(be, A, nvec, sr, om, si=None, with_b=True, seed=0, view=None):
+    """one device sweep and the oracle on the same inputs; returns (device, oracle) arrays"""
+    n = A.nrows
+    rng = np.random.default_rng(seed)
+    dt = be.dtype
+    x0 = rng.uniform(-1, 1, (n, nvec)).astype(dt)
+    xi0 = rng.uniform(-1, 1, (n, nvec)).astype(dt) if si is not None else None
+    b0 = rng.uniform(-1, 1, (n, nvec)).astype(dt) if with_b else None
+    m = A.range_map
+    objs = []
+    if view is None:
+        X = be.mvec(m, nvec, x0)
+        Xi = be.mvec(m, nvec, xi0) if si is not None else None
+        objs += [X] + ([Xi] if Xi is not None else [])
+    else:  # columns jmin..jmin+nvec-1 of a wider block
+        lo, width = view
+        big = rng.uniform(-1, 1, (n, width)).astype(dt)
+        big[:, lo:lo + nvec] = x0
+        R = be.mvec(m, width, big)
+        X = R.view(lo, lo + nvec - 1)
+        Xi = None
+        objs += [X, R]
+    B = be.mvec(m, nvec, b0) if with_b else None
+    if B is not None:
+        objs.append(B)
+    W = be.carp_setup(A, sr, si)
+    cp, rows = W.colors()
+    W.sweep(sr, B, X, om, si, Xi)
+    got = X.get()
+    goti = Xi.get() if Xi is not None else None
+    if view is not None:
+        full = R.get()
+        assert np.array_equal(full[:, :lo], big[:, :lo]) and np.array_equal(full[:, lo + nvec:], big[:, lo + nvec:])
+    W.delete()
+    for o in objs:
+        o.delete()
+    ro, ci, va = A.crs()
+    check_coloring(cp, rows, ro, ci, n)
+    xr, xi = x0.copy(), (xi0.copy() if xi0 is not None else None)
+    ol.kacz_rows(ro, ci, va, np.asarray(sr, dt), np.asarray(om, dt), xr, colour_sequence(cp, rows), b=b0,
+                 si=None if si is None else np.asarray(si, dt), xi=xi)
+    return got, goti, xr, xi, len(cp) - 1
+
+
+@pytest.mark.parametrize("mat", MATS)
+@pytest.mark.parametrize("nvec", [1, 4])
+def test_sweep_bitexact_fp64(be64, mat, nvec):
+    A = be64.read_mm(mm(mat))
+    rng = np.random.default_rng(nvec)
+    sr = rng.uniform(-1, 1, nvec)
+    om = rng.uniform(0.5, 1.9, nvec)
+    got, _, want, _, nc = run_pair(be64, A, nvec, sr, om, seed=MATS.index(mat))
+    assert np.array_equal(got, want), f"{mat}: max diff {np.abs(got - want).max()} ({nc} colours)"
+    A.delete()
+
+
+@pytest.mark.parametrize("mat", ["Dsprandn388", "DjadaTestMat512"])
+def test_sweep_rc_bitexact(be64, mat):
+    A = be64.read_mm(mm(mat))
+    nvec = 3
+    sr, si, om = np.array([0.3, -0.7, 1.2]), np.array([0.9, -0.2, 0.05]), np.array([1.0, 1.5, 0.8])
+    got, goti, want, wanti, _ = run_pair(be64, A, nvec, sr, om, si=si, seed=3)
+    assert np.array_equal(got, want) and np.array_equal(goti, wanti)
+    A.delete()
+
+
+def test_sweep_b_null(be64):
+    A = be64.read_mm(mm("Dsprandsym163"))
+    got, _, want, _, _ = run_pair(be64, A, 2, [0.1, 0.4], [1.2, 1.2], with_b=False, seed=5)
+    assert np.array_equal(got, want)
+    A.delete()
+
+
+def test_sweep_more_than_64_columns(be64):
+    A = be64.read_mm(mm("Dsprandn25"))
+    rng = np.random.default_rng(70)
+    sr, om = rng.uniform(-1, 1, 70), rng.uniform(0.5, 1.9, 70)
+    got, _, want, _, _ = run_pair(be64, A, 70, sr, om, seed=7)
+    assert np.array_equal(got, want)
+    A.delete()
+
+
+def test_sweep_on_column_view(be64):
+    A = be64.read_mm(mm("Dsprandn388"))
+    got, _, want, _, _ = run_pair(be64, A, 3, [0.2, 0.5, -0.3], [1.0, 1.3, 1.7], seed=9, view=(2, 8))
+    assert np.array_equal(got, want)
+    A.delete()
+
+
+@pytest.mark.parametrize("kind", ["laplace", "honeycomb", "banded"])
+def test_sweep_generated_matrices(be64, kind):
+    if kind == "laplace":
+        A = be64.laplace3d(12, 10, 9)
+    elif kind == "honeycomb":
+        A = be64.honeycomb(24, 20, 1, 1, 1, 0.3, 11)
+    else:
+        A = be64.banded_random(20000, 16, 300, 40.0, 5)
+    got, _, want, _, nc = run_pair(be64, A, 2, [0.5, -0.25], [1.4, 1.0], seed=11)
+    assert np.array_equal(got, want), f"{kind}: {nc} colours"
+    A.delete()
+
+
+def test_sweep_fp32_bitexact(be32):
+    A = be32.laplace3d(10, 9, 8)
+    got, _, want, _, _ = run_pair(be32, A, 4, [0.5, 0.0, -1.0, 2.0], [1.0, 1.2, 1.5, 1.8], seed=13)
+    assert np.array_equal(got, want)
+    A.delete()
+
+
+def test_identity_yields_zero(be64):
+    """CarpTest_def.hpp:600-617"""
+    A = be64.read_mm(mm("Dspeye25"))
+    m = A.range_map
+    X = be64.mvec(m, 3, np.random.default_rng(1).uniform(-1, 1, (A.nrows, 3)))
+    B = be64.mvec(m, 3, np.zeros((A.nrows, 3)))
+    W = be64.carp_setup(A, np.zeros(3))
+    W.sweep(np.zeros(3), B, X, np.ones(3))
+    assert np.all(X.get() == 0.0)
+    W.delete()
+    for o in (X, B):
+        o.delete()
+    A.delete()
+
+
+def test_operator_is_deterministic(be64):
+    """CarpTest_def.hpp:517-560 (bitwise here, the reference asks 10 eps)"""
+    A = be64.banded_random(30000, 12, 500, 20.0, 3)
+    outs = []
+    for _ in range(2):
+        got, _, _, _, _ = run_pair(be64, A, 4, [0., 1., 2., 3.], [1.5] * 4, seed=17)
+        outs.append(got)
+    assert np.array_equal(outs[0], outs[1])
+    A.delete()
+
+
+def test_convergence_to_solution(be64):
+    """repeated sweeps converge to the solution of (A - sigma I) x = b (0 < omega < 2)"""
+    A = be64.laplace3d(8, 8, 8)
+    n = A.nrows
+    ro, ci, va = A.crs()
+    a = np.zeros((n, n))
+    for i in range(n):
+        a[i, ci[ro[i] - 1:ro[i + 1] - 1] - 1] = va[ro[i] - 1:ro[i + 1] - 1]
+    sr = np.array([-0.5, -2.0])  # A - sigma I positive definite
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal((n, 2))
+    xs = np.stack([np.linalg.solve(a - sr[c] * np.eye(n), b[:, c]) for c in range(2)], axis=1)
+    m = A.range_map
+    X, B = be64.mvec(m, 2, np.zeros((n, 2))), be64.mvec(m, 2, b)
+    W = be64.carp_setup(A, sr)
+    for _ in range(300):
+        W.sweep(sr, B, X, [1.5, 1.5])
+    err = np.abs(X.get() - xs).max() / np.abs(xs).max()
+    assert err < 1e-8, err
+    W.delete()
+    for o in (X, B):
+        o.delete()
+    A.delete()
+
+
+def test_augmented_variants_not_implemented(be64):
+    """builtin/carp_def.hpp returns -99 for carp_sweep_aug(_rc) as well"""
+    A = be64.read_mm(mm("Dsprandn25"))
+    fl = ctypes.c_int(0)
+    be64.L.phist_Dcarp_sweep_aug(A.h, None, None, None, None, None, None, None, ctypes.byref(fl))
+    assert fl.value == -99
+    be64.L.phist_Dcarp_sweep_aug_rc(A.h, None, None, None, None, None, None, None, None, None, None,
+                                    ctypes.byref(fl))
+    assert fl.value == -99
+    A.delete()
+
+
+def rowfunc_of(rows):
+    import phist_amd
+
+    @phist_amd.ROWFUNC
+    def rf(i, nnz, cols, vals, arg):
+        a, b = rows.row_offset[i] - 1, rows.row_offset[i + 1] - 1
+        nnz[0] = b - a
+        dv = ctypes.cast(vals, ctypes.POINTER(ctypes.c_double))
+        for k in range(a, b):
+            cols[k - a] = rows.col_idx[k] - 1
+            dv[k - a] = rows.val[k]
+        return 0
+    return rf
+
+
+def test_sweep_empty_rows(be64):
+    rows = ol.crs_from_rows(6, 6, [([], []), ([0, 5], [1.0, 2.0]), ([], []), ([3], [4.0]), ([], []), ([], [])])
+    rf = rowfunc_of(rows)
+    A = be64.from_rowfunc(6, 2, rf, None)
+    got, _, want, _, _ = run_pair(be64, A, 2, [0.0, 0.6], [1.0, 1.3], seed=19)
+    assert np.array_equal(got, want)
+    A.delete()
+
+
+def test_setup_rejects_rectangular(be64):
+    import phist_amd
+    rows = ol.crs_from_rows(5, 8, [([0], [1.0]), ([1, 7], [1.0, 2.0]), ([2], [1.0]), ([3], [1.0]), ([4], [1.0])])
+    rf = rowfunc_of(rows)
+    A = be64.from_rowfunc(5, 2, rf, None, ncols=8)
+    with pytest.raises(phist_amd.PhistError):
+        be64.carp_setup(A, [0.0])
+    A.delete()
+
+
+def _carp_cg(be, A, sr, si, B, tol, max_iter, omega):
+    """CARP-CG (CGMN) on the C ABI, the iteration of src/krylov/phist_carp_cg_def.hpp:253-430 without its
+    correction steps: CG on (I - Q) x = R b, where one sweep maps x to Q x + R b.  Complex shifts use
+    separate real and imaginary blocks (the reference's x_mvec).  Returns (x, iterations, ||b - (A-sI)x||/||b||)."""
+    n, k = A.nrows, len(sr)
+    m = A.range_map
+    rc = si is not None
+    W = be.carp_setup(A, sr, si)
+    blocks = {nm: [be.mvec(m, k, np.zeros((n, k)))] + ([be.mvec(m, k, np.zeros((n, k)))] if rc else [])
+              for nm in ("x", "r", "p", "q")}
+    om = np.full(k, omega)
+
+    def sweep(v, b):
+        W.sweep(sr, b, v[0], om, si, v[1] if rc else None)
+
+    def axpby(a, v, b, w):  # w = a v + b w, real a and b
+        for s, t in zip(v, w):
+            be.call("mvec_add_mvec", be.s(a), s.h, be.s(b), t.h)
+
+    def cdot(v, w):  # sum conj(v) w over the real/imaginary pair
+        re = be.dot(v[0], w[0]) + (be.dot(v[1], w[1]) if rc else 0.0)
+        im = (be.dot(v[0], w[1]) - be.dot(v[1], w[0])) if rc else np.zeros(k)
+        return re + 1j * im
+
+    def vadd(al, v, w):  # w += diag(al) v
+        a = np.ascontiguousarray(al, np.float64)
+        be.call("mvec_vadd_mvec", a.ctypes.data_as(ctypes.c_void_p), v.h, be.s(1.0), w.h)
+
+    def caxpy(al, v, w):  # w += al v with complex al per column
+        vadd(al.real, v[0], w[0])
+        if rc:
+            vadd(al.real, v[1], w[1])
+            vadd(-al.imag, v[1], w[0])
+            vadd(al.imag, v[0], w[1])
+
+    def resid():
+        xr = blocks["x"][0].get()
+        xi = blocks["x"][1].get() if rc else 0.0
+        return np.linalg.norm(b_np - (a_sp @ (xr + 1j * xi) - (xr + 1j * xi) * (sr + 1j * (si if rc else 0))),
+                              axis=0) / np.linalg.norm(b_np, axis=0)
+
+    ro, ci, va = A.crs()
+    import scipy.sparse as sp
+    a_sp = sp.csr_matrix((va, ci - 1, ro - 1), shape=(n, n))
+    b_np = B.get()
+    x, r, p, q = blocks["x"], blocks["r"], blocks["p"], blocks["q"]
+    axpby(1.0, x, 0.0, r)
+    sweep(r, B)
+    axpby(-1.0, x, 1.0, r)
+    axpby(1.0, r, 0.0, p)
+    r2 = cdot(r, r).real
+    it, rel = 0, resid()
+    for it in range(1, max_iter + 1):
+        axpby(1.0, p, 0.0, q)
+        sweep(q, None)
+        axpby(1.0, p, -1.0, q)  # q = (I - Q) p
+        alpha = r2 / cdot(p, q)
+        caxpy(alpha, p, x)
+        caxpy(-alpha, q, r)
+        r2_new = cdot(r, r).real
+        beta = r2_new / r2
+        r2 = r2_new
+        bt = np.ascontiguousarray(beta, np.float64)
+        for t, s in zip(p, r):  # p = r + beta p
+            be.call("mvec_vscale", t.h, bt.ctypes.data_as(ctypes.c_void_p))
+            be.call("mvec_add_mvec", be.s(1.0), s.h, be.s(1.0), t.h)
+        if it % 10 == 0:
+            rel = resid()
+            if np.all(rel < tol):
+                break
+    xout = x[0].get() + (1j * x[1].get() if rc else 0)
+    W.delete()
+    for v in blocks.values():
+        for o in v:
+            o.delete()
+    return xout, it, rel
+
+
+def test_carp_cg_graphene(be64):
+    """drivers/integration_tests.cmake:46: phist_carp_cg graphene128 none 1 1 1e-12 350 1.0 -- a 128x128
+    sheet with long-range hopping and on-site disorder 0.2, no shift, one right-hand side"""
+    A = be64.honeycomb(128, 128, 0, 0, 1, 0.2, 1)
+    n = A.nrows
+    B = be64.mvec(A.range_map, 1, np.random.default_rng(4).uniform(-1, 1, (n, 1)))
+    x, it, rel = _carp_cg(be64, A, np.zeros(1), None, B, 1e-12, 350, 1.0)
+    print(f"CARP-CG graphene128: {it} iterations, relres {rel}")
+    assert np.all(rel < 1e-12), (it, rel)
+    B.delete()
+    A.delete()
+
+
+def test_carp_cg_spin_imaginary_shifts(be64):
+    """drivers/integration_tests.cmake:51-52: phist_carp_cg spinSZ14 imaginary_shifts.txt 1 1 1e-6 350 1.2,
+    shifts 0.01i, 0.025i, 0.05i, 0.1i (exampleRuns/imaginary_shifts.txt), real and imaginary blocks"""
+    A = be64.spinchain_sz(14)
+    n = A.nrows
+    si = np.array([0.01, 0.025, 0.05, 0.1])
+    sr = np.zeros(4)
+    B = be64.mvec(A.range_map, 4, np.random.default_rng(5).uniform(-1, 1, (n, 4)))
+    x, it, rel = _carp_cg(be64, A, sr, si, B, 1e-6, 350, 1.2)
+    print(f"CARP-CG spinSZ14 imaginary shifts: {it} iterations, relres {rel}")
+    assert np.all(rel < 1e-6), (it, rel)
+    B.delete()
+    A.delete()
