@@ -82,3 +82,60 @@ def test_serve_budget(be64):
     for o in (B, X, Q, R, A):
         o.delete()
     srv.delete()
+
+
+def test_serve_result_same_alone_and_in_the_mix(be64):
+    """each request's result served alone equals its result served among others, within the solver
+    tolerance: solutions to 100*tol relative (the shifted systems have condition number < 30), equal
+    iteration counts to +-2, eigenvalues to 1e-9"""
+    A1 = be64.laplace3d(11, 12, 13)
+    A2 = be64.spinchain_sz(12, 6)
+    rng = np.random.default_rng(9)
+    specs = []
+    for A, nrhs, fp32 in [(A1, 3, False), (A2, 5, False), (A1, 4, True), (A2, 2, False), (A1, 6, False)]:
+        n = A.nrows
+        shifts = rng.uniform(0.5, 2.0, nrhs) + (0.0 if A is A1 else 12.0)
+        specs.append((A, shifts, rng.uniform(-1, 1, (n, nrhs)), 1e-5 if fp32 else 1e-10, fp32))
+
+    def serve(which, with_jada):
+        srv = be64.server(budget_bytes=1 << 30, slots=5)
+        out, objs = {}, []
+        for i in which:
+            A, shifts, Bh, tol, fp32 = specs[i]
+            B, X = be64.mvec(A.domain_map, len(shifts), Bh), be64.mvec(A.domain_map, len(shifts))
+            objs += [B, X]
+            out[i] = (srv.submit_solve(A, shifts, B, X, tol, 3000, fp32=fp32), X)
+        jada = srv.submit_jada(A2, 4, "SR", 1e-8, block_size=2, min_bas=12, max_bas=24) if with_jada else None
+        n = 0
+        while srv.step() > 0:
+            n += 1
+            assert n < 40000
+        res = {}
+        for i, (rid, X) in out.items():
+            st, _, it = srv.status(rid)
+            assert st == pa.Server.DONE
+            res[i] = (X.get(), it)
+        ev = None
+        if jada is not None:
+            jid, Q, R, ev_a, _ = jada
+            assert srv.status(jid)[0] == pa.Server.DONE
+            ev = np.sort(ev_a[:4].copy())
+            objs += [Q, R]
+        for o in objs:
+            o.delete()
+        srv.delete()
+        return res, ev
+
+    mixed, ev_mixed = serve(range(len(specs)), True)
+    for i in range(len(specs)):
+        alone, _ = serve([i], False)
+        xa, ita = alone[i]
+        xm, itm = mixed[i]
+        tol = specs[i][3]
+        rel = np.linalg.norm(xm - xa, axis=0) / np.linalg.norm(xa, axis=0)
+        assert np.all(rel <= 100 * tol), (i, rel)
+        assert abs(ita - itm) <= 2, (i, ita, itm)
+    _, ev_alone = serve([], True)
+    assert np.all(np.abs(ev_mixed - ev_alone) <= 1e-9 * np.abs(ev_alone).max())
+    for o in (A1, A2):
+        o.delete()
