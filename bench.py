@@ -525,6 +525,11 @@ def main():
     be.init()
     L = be.L
     iflag = ctypes.c_int(0)
+    if args.copy_calibration:
+        mean_bw, max_bw = ctypes.c_double(), ctypes.c_double()
+        L.phist_bench_stream_copy(ctypes.byref(mean_bw), ctypes.byref(max_bw), ctypes.byref(iflag))
+        print(f"bench: STREAM copy {mean_bw.value:.0f} GB/s mean, {max_bw.value:.0f} GB/s max",
+              file=sys.stderr, flush=True)
     if args.op == "jada":
         return jada_bench(be, args)
     if args.op == "minres":
@@ -537,11 +542,6 @@ def main():
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
     N, nvec = args.rows, args.nvec
-    if args.copy_calibration:
-        mean_bw, max_bw = ctypes.c_double(), ctypes.c_double()
-        L.phist_bench_stream_copy(ctypes.byref(mean_bw), ctypes.byref(max_bw), ctypes.byref(iflag))
-        print(f"bench: STREAM copy {mean_bw.value:.0f} GB/s mean, {max_bw.value:.0f} GB/s max",
-              file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     A = be.banded_random(N, 32, 50000, 40.0, 2024)
     t_build = time.perf_counter() - t0

@@ -95,6 +95,7 @@ struct Runtime {
   int gram_blocks_per_cu = 8;
   // row-mixing products (V*C, in-place V*M) with k, m >= 8 run on the matrix cores
   bool mix_mfma = true;
+  bool minres_device = true;  // MINRES scalars on the device for this library's matrices
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;
@@ -107,6 +108,18 @@ void ensure_init();
 void* device_work(size_t bytes);
 void* host_work(size_t bytes);
 void sync_stream();
+
+// Caching allocators for block vectors and small dense matrices.  Released
+// device blocks go back to a size-class cache and are handed out again
+// without synchronising: every use of a block is ordered on the library
+// stream.  Pinned host blocks are cached after a stream synchronisation
+// (a copy from or into them may still be queued).  Exceeding the cache caps
+// or a failed allocation returns cached blocks to the runtime.
+void* dev_alloc(size_t bytes);
+void dev_release(void* p, size_t bytes);
+void* pinned_alloc(size_t bytes);
+void pinned_release(void* p, size_t bytes);
+void pool_trim();
 // brackets one kernel launch when the kernel timer is on
 void ktimer_begin();
 void ktimer_end();  // synchronises the event and accumulates the time
@@ -237,6 +250,15 @@ struct SpmvRequest {
   bool has_shift = false, has_beta = false;
   bool dot_ww = false, dot_vw = false;
   bool axpy_u = false;  // U = gamma*W + delta*U after W is formed
+  // when set, the dots stay on the device: dots_dev[0..nvec) = ww, dots_dev[nvec..2nvec) = vw
+  // (both written whichever were requested; at most 64 column groups)
+  void* dots_dev = nullptr;
+};
+
+// per-column state of the device-resident MINRES recurrences (Paige & Saunders)
+struct MinresState {
+  double beta1, beta, c1, s1, c2, s2, phibar, relres, tol;
+  int iters, max_iter, active, pad;
 };
 
 // y = alpha*(A + diag-shifts)*x + beta*y over column range of the given views;
@@ -265,7 +287,9 @@ template <typename T> void carp_colors(void* work, int* ncolors, int64_t* color_
 
 // tall-skinny GEMMs; the sdMat results are written to the device copy and
 // copied to the host copy before returning
-template <typename T> void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C);
+// to_host = false leaves C only on the device (its host mirror is not refreshed)
+template <typename T>
+void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C, bool to_host = true);
 template <typename T> void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W);
 template <typename T> void gemm_times_sd_inplace(Mvec<T>* V, const SdMat<T>* M);
 template <typename T>

@@ -175,6 +175,8 @@ class JadaRun {
     t_.p = api::mvec_create<T>(map, bs);
     res_.p = api::mvec_create<T>(map, std::min(lookAhead, nEigMax));
     H_.p = api::sdMat_create<T>(maxBase, maxBase);
+    solve_.resize(bs + 1);
+    solve32_.resize(bs + 1);
     theta.assign(maxBase, 0.0);
     resNorm.assign(nEigMax, -1.0);
     resScaling.assign(nEigMax, 1.0);
@@ -374,11 +376,13 @@ class JadaRun {
         blas_copy<T>(rhs.get(), t.get());
       } else if (A32) {
         for (double& x : tl) x = std::max(x, 1.0e-6);  // FP32 solves stagnate below this
-        MinresOut<float> r = minres_solve<T, float>(*A32, s, Qtil.get(), rhs.get(), t.get(), tl, innerMaxIters);
+        if (!solve32_[k]) solve32_[k].reset(new MinresSolver<T, float>(*A32, Qout->map, k));
+        MinresOut<float> r = solve32_[k]->solve(s, Qtil.get(), rhs.get(), t.get(), tl, innerMaxIters);
         inner_it = r.iters;
         inner_rr = r.relres;
       } else {
-        MinresOut<T> r = minres_solve<T, T>(A, s, Qtil.get(), rhs.get(), t.get(), tl, innerMaxIters);
+        if (!solve_[k]) solve_[k].reset(new MinresSolver<T, T>(A, Qout->map, k));
+        MinresOut<T> r = solve_[k]->solve(s, Qtil.get(), rhs.get(), t.get(), tl, innerMaxIters);
         inner_it = r.iters;
         inner_rr = r.relres;
       }
@@ -455,6 +459,9 @@ class JadaRun {
   const int nEigMax;
   MvBox<T> V_, AV_, Q_, t_, res_;
   SdBox<T> H_;
+  // correction-equation solvers by number of systems (kept between outer iterations)
+  std::vector<std::unique_ptr<MinresSolver<T, T>>> solve_;
+  std::vector<std::unique_ptr<MinresSolver<T, float>>> solve32_;
   int nV = 0, nConv = 0, iter = 0;
   bool done_ = false;
   std::vector<double> theta, resNorm, resScaling, innerTol, lastOuterRes, hb, qh;

@@ -2,7 +2,8 @@
 // precision conversion of block vectors.
 //
 // MINRES (Paige & Saunders) on k independent shifted systems at once, one
-// column per system, the per-column scalars kept on the host:
+// column per system, the per-column scalars on the host (kernel arguments) or,
+// for krylov_minres_step_dev, on the device (k_minres_ab, k_minres_rot):
 //
 //   k_lanczos    w(:,c) -= a_c * v(:,c) + b_c * vold(:,c), and the column
 //                norms of the new w (fixed-order two-stage reduction)
@@ -34,8 +35,24 @@ struct KrylovArgs {
   int64_t ld;  // common row stride of every block (elements)
   int flags;
   T* partials;
+  // coefficients: from cdev (device, coefficient q of column c at cdev[q*ncols + c]) when it is set,
+  // else from the kernel arguments c0..c4
+  const T* cdev;
   T c0[kKrylovMaxCols], c1[kKrylovMaxCols], c2[kKrylovMaxCols], c3[kKrylovMaxCols], c4[kKrylovMaxCols];
 };
+
+template <typename T>
+__device__ inline T kcoef(const KrylovArgs<T>& p, int q, int c) {
+  if (c >= p.ncols) return T(0);
+  if (p.cdev) return p.cdev[q * p.ncols + c];
+  switch (q) {
+    case 0: return p.c0[c];
+    case 1: return p.c1[c];
+    case 2: return p.c2[c];
+    case 3: return p.c3[c];
+    default: return p.c4[c];
+  }
+}
 
 enum : int { KF_VOLD = 1, KF_D2 = 2 };
 
@@ -48,8 +65,8 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
   T a[VPL], b[VPL], acc[VPL];
 #pragma unroll
   for (int e = 0; e < VPL; e++) {
-    a[e] = c0 + e < p.ncols ? p.c0[c0 + e] : T(0);
-    b[e] = c0 + e < p.ncols ? p.c1[c0 + e] : T(0);
+    a[e] = kcoef(p, 0, c0 + e);
+    b[e] = kcoef(p, 1, c0 + e);
     acc[e] = T(0);
   }
   const TileWalk tw = xcd_tiles(p.ntiles);
@@ -97,12 +114,11 @@ __global__ void __launch_bounds__(kBlock) k_minres(const KrylovArgs<T> p) {
   T dl[VPL], ep[VPL], ig[VPL], ta[VPL], ib[VPL];
 #pragma unroll
   for (int e = 0; e < VPL; e++) {
-    const bool ok = c0 + e < p.ncols;
-    dl[e] = ok ? p.c0[c0 + e] : T(0);
-    ep[e] = ok ? p.c1[c0 + e] : T(0);
-    ig[e] = ok ? p.c2[c0 + e] : T(0);
-    ta[e] = ok ? p.c3[c0 + e] : T(0);
-    ib[e] = ok ? p.c4[c0 + e] : T(0);
+    dl[e] = kcoef(p, 0, c0 + e);
+    ep[e] = kcoef(p, 1, c0 + e);
+    ig[e] = kcoef(p, 2, c0 + e);
+    ta[e] = kcoef(p, 3, c0 + e);
+    ib[e] = kcoef(p, 4, c0 + e);
   }
   const TileWalk tw = xcd_tiles(p.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
@@ -143,6 +159,65 @@ __global__ void __launch_bounds__(kBlock) k_convert(const TS* __restrict__ x, in
     const int c = (int)(e - i * ncols);
     y[i * ldy + c] = (TD)x[i * ldx + c];
   }
+}
+
+// Lanczos coefficients of the active columns: a_c = <v, Av>_c (the fused SpMV
+// dots, vw half), b_c = beta_c after the first iteration; 0 for frozen columns
+template <typename T>
+__global__ void k_minres_ab(const T* __restrict__ dots, const MinresState* __restrict__ st, int k, T* __restrict__ ab) {
+  const int c = threadIdx.x;
+  if (c >= k) return;
+  const MinresState q = st[c];
+  ab[c] = q.active ? dots[k + c] : T(0);
+  ab[k + c] = (q.active && q.iters > 0) ? (T)q.beta : T(0);
+}
+
+// One step of the Paige & Saunders recurrences per active column, in FP64:
+// column `iters` of the Lanczos tridiagonal (beta_k, alpha_k, beta_{k+1})
+// meets the two previous rotations, a new rotation annihilates beta_{k+1};
+// writes the k_minres coefficients (delta, eps, 1/gamma, tau, 1/beta_{k+1})
+// and freezes columns that converged, broke down or hit their limit.
+template <typename T>
+__global__ void k_minres_rot(const T* __restrict__ dots, const T* __restrict__ nrm2, MinresState* __restrict__ st,
+                             int k, T* __restrict__ coef) {
+  const int c = threadIdx.x;
+  if (c >= k) return;
+  MinresState q = st[c];
+  T dl = T(0), ep = T(0), ig = T(0), ta = T(0), ib = T(0);
+  if (q.active) {
+    const T bnext = sqrt(nrm2[c]);
+    const double bk = q.iters > 0 ? q.beta : 0.0, ak = (double)dots[k + c], bn = (double)bnext;
+    const double eps_k = q.s2 * bk, dbar = q.c2 * bk;
+    const double delta = q.c1 * dbar + q.s1 * ak;
+    const double gbar = -q.s1 * dbar + q.c1 * ak;
+    const double gamma = hypot(gbar, bn);
+    if (!(gamma > 0.0) || !isfinite(gamma)) {
+      q.active = 0;
+    } else {
+      const double cs = gbar / gamma, sn = bn / gamma;
+      const double tau = cs * q.phibar;
+      q.phibar = -sn * q.phibar;
+      dl = (T)delta;
+      ep = (T)eps_k;
+      ig = (T)(1.0 / gamma);
+      ta = (T)tau;
+      ib = bn > 0.0 ? (T)(1.0 / bn) : T(0);
+      q.c2 = q.c1;
+      q.s2 = q.s1;
+      q.c1 = cs;
+      q.s1 = sn;
+      q.beta = bn;
+      q.iters++;
+      q.relres = fabs(q.phibar) / q.beta1;
+      if (q.relres <= q.tol || bn == 0.0 || q.iters >= q.max_iter) q.active = 0;
+    }
+    st[c] = q;
+  }
+  coef[c] = dl;
+  coef[k + c] = ep;
+  coef[2 * k + c] = ig;
+  coef[3 * k + c] = ta;
+  coef[4 * k + c] = ib;
 }
 
 namespace {
@@ -211,6 +286,7 @@ void krylov_lanczos(int64_t nrows, int ncols, int64_t ld, const T* v, const T* v
   p.vold = const_cast<T*>(vold);
   p.w = w;
   p.flags = 0;
+  p.cdev = nullptr;
   PHX_REQUIRE(ncols <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns too wide", ncols);
   for (int c = 0; c < ncols; c++) {
     p.c0[c] = a[c];
@@ -239,6 +315,7 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
   p.d2 = d2;
   p.x = x;
   p.flags = 0;
+  p.cdev = nullptr;
   PHX_REQUIRE(ncols <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns too wide", ncols);
   for (int c = 0; c < ncols; c++) {
     p.c0[c] = delta[c];
@@ -248,6 +325,45 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
     p.c4[c] = inv_beta[c];
     if (eps[c] != T(0)) p.flags |= KF_D2;
   }
+  launch(true, p, {v, vold, w, d1, d2, x});
+}
+
+// Device-resident variant of one MINRES iteration's vector work and scalars
+// (no host synchronisation): given the fused SpMV dots (dots[k..2k) = <v, Av>)
+// and the column states st, it forms w -= a.*v + b.*vold with the new norms,
+// advances the recurrences on the device and updates d2, x and vold.
+// work: device scratch of 7*k values of T (a, b, then the five k_minres
+// coefficients) plus k squared norms.
+template <typename T>
+void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
+                            const T* dots, MinresState* st, T* work) {
+  PHX_REQUIRE(k >= 1 && k <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns", k);
+  if (nrows == 0) return;  // every column starts frozen (zero right-hand side)
+  hipStream_t s = rt().stream;
+  T* ab = work;
+  T* coef = work + 2 * k;
+  T* nrm2 = work + 7 * k;
+  hipLaunchKernelGGL(k_minres_ab<T>, dim3(1), dim3(64), 0, s, dots, st, k, ab);
+  PHX_HIP(hipGetLastError());
+  KrylovArgs<T> p;
+  p.nrows = nrows;
+  p.ncols = k;
+  p.ld = ld;
+  p.v = const_cast<T*>(v);
+  p.vold = vold;
+  p.w = w;
+  p.d1 = const_cast<T*>(d1);
+  p.d2 = d2;
+  p.x = x;
+  p.flags = KF_VOLD;
+  p.cdev = ab;
+  const int grid = launch(false, p, {v, w, vold});
+  hipLaunchKernelGGL(k_reduce_partials<T>, dim3((k + 15) / 16), dim3(kBlock), 0, s, p.partials, grid, k, nrm2);
+  PHX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_minres_rot<T>, dim3(1), dim3(64), 0, s, dots, nrm2, st, k, coef);
+  PHX_HIP(hipGetLastError());
+  p.flags = KF_D2;
+  p.cdev = coef;
   launch(true, p, {v, vold, w, d1, d2, x});
 }
 
@@ -272,6 +388,10 @@ template void krylov_minres_update<double>(int64_t, int, int64_t, const double*,
 template void krylov_minres_update<float>(int64_t, int, int64_t, const float*, float*, const float*, const float*,
                                           float*, float*, const float*, const float*, const float*, const float*,
                                           const float*);
+template void krylov_minres_step_dev<double>(int64_t, int, int64_t, const double*, double*, double*, const double*,
+                                             double*, double*, const double*, MinresState*, double*);
+template void krylov_minres_step_dev<float>(int64_t, int, int64_t, const float*, float*, float*, const float*, float*,
+                                            float*, const float*, MinresState*, float*);
 template void convert_block<double, float>(const double*, int64_t, float*, int64_t, int64_t, int);
 template void convert_block<float, double>(const float*, int64_t, double*, int64_t, int64_t, int);
 

@@ -21,7 +21,7 @@ T* Mvec<T>::host() {
   Mvec* r = root;
   if (!r->host_base) {
     size_t n = (size_t)std::max<int64_t>(1, r->nrows * r->lda);
-    PHX_HIP(hipHostMalloc((void**)&r->host_base, n * sizeof(T), hipHostMallocDefault));
+    r->host_base = static_cast<T*>(pinned_alloc(n * sizeof(T)));
     memset(r->host_base, 0, n * sizeof(T));
   }
   return r->host_base + jofs;
@@ -40,10 +40,11 @@ Mvec<T>* mvec_create(const Map* map, int nvec) {
   V->nvec = nvec;
   V->lda = nvec;
   const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(1, V->nrows * nvec);
-  hipError_t e = hipMalloc((void**)&V->dev_base, bytes);
-  if (e != hipSuccess) {
+  try {
+    V->dev_base = static_cast<T*>(dev_alloc(bytes));
+  } catch (...) {
     delete V;
-    fail(PHIST_MEM_ALLOC_FAILED, "hipMalloc of %zu bytes for an mvec failed", bytes);
+    throw;
   }
   PHX_HIP(hipMemsetAsync(V->dev_base, 0, bytes, rt().stream));
   return V;
@@ -57,9 +58,9 @@ void mvec_delete(Mvec<T>* V) {
     delete V;
     return;
   }
-  sync_stream();
-  if (V->dev_base) PHX_HIP(hipFree(V->dev_base));
-  if (V->host_base) PHX_HIP(hipHostFree(V->host_base));
+  const size_t n = (size_t)std::max<int64_t>(1, V->nrows * V->lda);
+  if (V->dev_base) dev_release(V->dev_base, n * sizeof(T));
+  if (V->host_base) pinned_release(V->host_base, n * sizeof(T));
   delete V;
 }
 
@@ -199,19 +200,24 @@ void mvec_set_data(Mvec<T>* V, const T* data, int64_t lda, int row_major) {
   }
 }
 
+// The element values are staged in a packed host array and uploaded: a column
+// view of a wide block does not need the root's pinned host mirror (which
+// for a basis of many columns is gigabytes of page-locked memory).
 template <typename T>
 void mvec_put_func(Mvec<T>* V, phist_mvec_elemFunc f, void* arg) {
   PHX_REQUIRE(f, PHIST_INVALID_INPUT, "NULL element function");
-  T* h = V->host();
+  if (V->nrows == 0) return;
+  std::vector<T> h((size_t)V->nrows * V->nvec);
   const int64_t il = V->map->ilower;
   for (int64_t i = 0; i < V->nrows; i++)
-    for (int j = 0; j < V->nvec; j++) f((phist_gidx)(il + i), (phist_lidx)j, (void*)&h[i * V->lda + j], arg);
-  mvec_to_device(V);
+    for (int j = 0; j < V->nvec; j++) f((phist_gidx)(il + i), (phist_lidx)j, (void*)&h[(size_t)i * V->nvec + j], arg);
+  copy_rows_h2d(V->dev(), V->lda * sizeof(T), h.data(), V->nvec * sizeof(T), V->nvec * sizeof(T), V->nrows);
 }
 
 // Draws nrows*nvec values of the global stream (row-major) and then skips
 // (gnrows - iupper)*nvec = nvec values, as kernels_common_impl_def.h:256-361
 // does on a single process (pre_skip = ilower*nvec = 0).
+// The values are uploaded from a packed host array (no root host mirror).
 template <typename T>
 void mvec_random(Mvec<T>* V) {
   const int64_t n = V->nrows * V->nvec;
@@ -219,11 +225,11 @@ void mvec_random(Mvec<T>* V) {
   kiss_draw(n, r.data());
   const int64_t iupper = V->map->ilower + V->map->nloc - 1;
   kiss_skip((uint64_t)((V->map->nglob - iupper) * V->nvec));
-  T* h = V->host();
+  if (n == 0) return;
+  std::vector<T> h((size_t)n);
 #pragma omp parallel for schedule(static) if (n > (1 << 16))
-  for (int64_t i = 0; i < V->nrows; i++)
-    for (int j = 0; j < V->nvec; j++) h[i * V->lda + j] = (T)r[(size_t)i * V->nvec + j];
-  mvec_to_device(V);
+  for (int64_t e = 0; e < n; e++) h[(size_t)e] = (T)r[(size_t)e];
+  copy_rows_h2d(V->dev(), V->lda * sizeof(T), h.data(), V->nvec * sizeof(T), V->nvec * sizeof(T), V->nrows);
 }
 
 template <typename T>
@@ -275,9 +281,9 @@ SdMat<T>* sdMat_create(int nrows, int ncols) {
   M->ncols = ncols;
   M->ld = std::max(1, nrows);
   const size_t n = (size_t)M->ld * std::max(1, ncols);
-  PHX_HIP(hipHostMalloc((void**)&M->host_base, n * sizeof(T), hipHostMallocDefault));
+  M->host_base = static_cast<T*>(pinned_alloc(n * sizeof(T)));
   memset(M->host_base, 0, n * sizeof(T));
-  PHX_HIP(hipMalloc((void**)&M->dev_base, n * sizeof(T)));
+  M->dev_base = static_cast<T*>(dev_alloc(n * sizeof(T)));
   PHX_HIP(hipMemsetAsync(M->dev_base, 0, n * sizeof(T), rt().stream));
   return M;
 }
@@ -290,9 +296,9 @@ void sdMat_delete(SdMat<T>* M) {
     delete M;
     return;
   }
-  sync_stream();
-  if (M->dev_base) PHX_HIP(hipFree(M->dev_base));
-  if (M->host_base) PHX_HIP(hipHostFree(M->host_base));
+  const size_t n = (size_t)M->ld * std::max(1, M->ncols);
+  if (M->dev_base) dev_release(M->dev_base, n * sizeof(T));
+  if (M->host_base) pinned_release(M->host_base, n * sizeof(T));
   delete M;
 }
 

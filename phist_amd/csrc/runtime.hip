@@ -7,6 +7,9 @@
 // 289-320 (contiguous distribution; ilower/iupper are 0-based).
 #include "amd_common.hpp"
 #include <cstdarg>
+#include <map>
+#include <mutex>
+#include <vector>
 #include <cstdlib>
 #include <algorithm>
 
@@ -49,6 +52,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_BLOCKS_PER_CU")) r.gram_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA")) r.mix_mfma = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_MINRES_DEVICE")) r.minres_device = atoi(s) != 0;
   kiss_reset();
   r.initialized = true;
 }
@@ -84,6 +88,129 @@ void* host_work(size_t bytes) {
 }
 
 void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
+
+namespace {
+// size classes: powers of two up to 2 MiB, then multiples of 2 MiB
+size_t size_class(size_t bytes) {
+  constexpr size_t kBig = size_t(2) << 20;
+  if (bytes <= 4096) return 4096;
+  if (bytes >= kBig) return (bytes + kBig - 1) / kBig * kBig;
+  size_t c = 4096;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+struct BlockCache {
+  std::map<size_t, std::vector<void*>> free;
+  size_t cached = 0, cap;
+  explicit BlockCache(size_t c) : cap(c) {}
+  void* take(size_t cls) {
+    auto it = free.find(cls);
+    if (it == free.end() || it->second.empty()) return nullptr;
+    void* p = it->second.back();
+    it->second.pop_back();
+    cached -= cls;
+    return p;
+  }
+};
+
+BlockCache& dev_cache() {
+  static BlockCache c(size_t(32) << 30);
+  return c;
+}
+BlockCache& pin_cache() {
+  static BlockCache c(size_t(1) << 30);
+  return c;
+}
+std::mutex& cache_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+void trim(BlockCache& c, bool device) {
+  if (c.cached == 0) return;
+  PHX_HIP(hipStreamSynchronize(rt().stream));
+  for (auto& kv : c.free)
+    for (void* p : kv.second) {
+      if (device)
+        PHX_HIP(hipFree(p));
+      else
+        PHX_HIP(hipHostFree(p));
+    }
+  c.free.clear();
+  c.cached = 0;
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  std::lock_guard<std::mutex> lk(cache_mutex());
+  if (void* p = dev_cache().take(cls)) return p;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, cls);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim(dev_cache(), true);
+    e = hipMalloc(&p, cls);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(PHIST_MEM_ALLOC_FAILED, "hipMalloc of %zu bytes failed", cls);
+  }
+  return p;
+}
+
+void dev_release(void* p, size_t bytes) {
+  if (!p) return;
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  std::lock_guard<std::mutex> lk(cache_mutex());
+  BlockCache& c = dev_cache();
+  if (c.cached + cls > c.cap) {
+    PHX_HIP(hipStreamSynchronize(rt().stream));
+    PHX_HIP(hipFree(p));
+    return;
+  }
+  c.free[cls].push_back(p);
+  c.cached += cls;
+}
+
+void* pinned_alloc(size_t bytes) {
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  std::lock_guard<std::mutex> lk(cache_mutex());
+  if (void* p = pin_cache().take(cls)) return p;
+  void* p = nullptr;
+  hipError_t e = hipHostMalloc(&p, cls, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim(pin_cache(), false);
+    e = hipHostMalloc(&p, cls, hipHostMallocDefault);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(PHIST_MEM_ALLOC_FAILED, "hipHostMalloc of %zu bytes failed", cls);
+  }
+  return p;
+}
+
+void pinned_release(void* p, size_t bytes) {
+  if (!p) return;
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  std::lock_guard<std::mutex> lk(cache_mutex());
+  PHX_HIP(hipStreamSynchronize(rt().stream));
+  BlockCache& c = pin_cache();
+  if (c.cached + cls > c.cap) {
+    PHX_HIP(hipHostFree(p));
+    return;
+  }
+  c.free[cls].push_back(p);
+  c.cached += cls;
+}
+
+void pool_trim() {
+  std::lock_guard<std::mutex> lk(cache_mutex());
+  trim(dev_cache(), true);
+  trim(pin_cache(), false);
+}
 
 void ktimer_begin() {
   Runtime& r = rt();
@@ -189,6 +316,7 @@ void phist_kernels_finalize(int* iflag) {
     Runtime& r = rt();
     if (r.initialized) {
       PHX_HIP(hipStreamSynchronize(r.stream));
+      pool_trim();
       if (r.d_work) PHX_HIP(hipFree(r.d_work));
       if (r.h_work) PHX_HIP(hipHostFree(r.h_work));
       PHX_HIP(hipEventDestroy(r.ev_start));

@@ -169,6 +169,16 @@ struct Operator {
     PHX_REQUIRE(iflag == 0, iflag, "user operator returned %d", iflag);
     if (xy_dots) blas_dot<T>(X, Y, xy_dots);
   }
+  // Y = (A + diag(s)) X with the column dots <X, Y> left on the device
+  // (dots_dev[nvec + c]; dots_dev[c] = <Y, Y>); sparse matrices of this library only
+  void apply_shifted_dev(const T* s, const Mvec<T>* X, Mvec<T>* Y, T* dots_dev) const {
+    PHX_REQUIRE(A != nullptr, PHIST_NOT_IMPLEMENTED, "device dots need a sparse matrix of this library");
+    SpmvRequest req;
+    req.has_shift = true;
+    req.dot_vw = true;
+    req.dots_dev = dots_dev;
+    spmv_launch<T>(A, T(1), s, X, T(0), Y, req, nullptr, nullptr);
+  }
 };
 
 
@@ -187,40 +197,82 @@ void copy_any(const Mvec<TS>* X, Mvec<TD>* Y) {
 // MINRES with Paige & Saunders' short recurrences.  Per iteration: one
 // shifted SpMV whose fused <v, Av> dots give the Lanczos alphas, the
 // projection, k_lanczos (update + norms) and k_minres (direction, solution,
-// next Lanczos vector); the 2x2 rotations of the slots run on the host.  A
-// slot whose system has converged (|phibar| <= tol*|b|), broken down or
-// reached its iteration limit stops updating; its solution can be stored and
-// the slot loaded with the next system while the others keep iterating (the
-// slot swap of phist_jadaCorrectionSolver_def.hpp:428-466).
+// next Lanczos vector).  A slot whose system has converged (|phibar| <=
+// tol*|b|), broken down or reached its iteration limit stops updating; its
+// solution can be stored and the slot loaded with the next system while the
+// others keep iterating (the slot swap of
+// phist_jadaCorrectionSolver_def.hpp:428-466).
+//
+// For a sparse matrix of this library the scalars live on the device
+// (krylov_minres_step_dev): an iteration is a chain of launches with no host
+// synchronisation, and the host reads the slot states one iteration late
+// (a copy into pinned memory per iteration, waited for after the next
+// iteration has been queued).  The accessors (active, iters, relres) report
+// that lagged state; a loop `while (any_active()) iterate();` therefore runs
+// one iteration more than needed, which changes nothing for frozen slots.
+// For a user operator the scalars stay on the host, as the dots come back
+// to the host anyway.
 // ---------------------------------------------------------------------------
+template <typename T>
+void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
+                            const T* dots, MinresState* st, T* work);
+
 template <typename T>
 class MinresBatch {
  public:
   MinresBatch(const Operator<T>& op, const Map* map, int cap, const Mvec<T>* Q = nullptr)
-      : op_(op), map_(map), cap_(cap), Q_(Q), work_(map, 6 * cap), st_(cap) {
+      : op_(op), map_(map), cap_(cap), Q_(Q), work_(map, 6 * cap), st_(cap), load_seq_(cap, 0) {
     PHX_REQUIRE(cap >= 1 && cap <= 64, PHIST_INVALID_INPUT, "MINRES batch of %d systems", cap);
     for (int b = 0; b < 6; b++) blk_[b].p = nullptr;
     for (int b = 0; b < 6; b++) api::mvec_view_block(work_.get(), &blk_[b].p, b * cap, (b + 1) * cap - 1);
     blas_put_value<T>(work_.get(), T(0));
     if (Q_ && Q_->nvec > 0) C_.reset(new SdBox<T>(Q_->nvec, cap));
+    dev_ = op_.A != nullptr && rt().minres_device;
+    if (dev_) {
+      d_st_ = static_cast<MinresState*>(dev_alloc(sizeof(MinresState) * cap));
+      d_dots_ = static_cast<T*>(dev_alloc(sizeof(T) * 2 * cap));
+      d_work_ = static_cast<T*>(dev_alloc(sizeof(T) * 8 * cap));
+      h_buf_ = static_cast<MinresState*>(pinned_alloc(sizeof(MinresState) * 3 * cap));
+      for (int e = 0; e < 3; e++) PHX_HIP(hipEventCreateWithFlags(&ev_[e], hipEventDisableTiming));
+      upload(0, cap);  // all frozen
+    }
+  }
+  ~MinresBatch() {
+    if (!dev_) return;
+    try {
+      for (int e = 0; e < 3; e++)
+        if (ev_[e]) PHX_HIP(hipEventDestroy(ev_[e]));
+      pinned_release(h_buf_, sizeof(MinresState) * 3 * cap_);  // synchronises the stream first
+      dev_release(d_work_, sizeof(T) * 8 * cap_);
+      dev_release(d_dots_, sizeof(T) * 2 * cap_);
+      dev_release(d_st_, sizeof(MinresState) * cap_);
+    } catch (...) {
+    }
   }
   MinresBatch(const MinresBatch&) = delete;
   MinresBatch& operator=(const MinresBatch&) = delete;
 
   int capacity() const { return cap_; }
+  // projector P = I - QQ' for the following systems (Q = null: none); Q must
+  // stay alive while the batch iterates
+  void set_projection(const Mvec<T>* Q) {
+    Q_ = (Q && Q->nvec > 0) ? Q : nullptr;
+    if (Q_ && (!C_ || C_->get()->nrows != Q_->nvec)) C_.reset(new SdBox<T>(Q_->nvec, cap_));
+    if (!Q_) C_.reset();
+  }
   int free_slot() const {
     for (int j = 0; j < cap_; j++)
       if (!st_[j].used) return j;
     return -1;
   }
   bool used(int j) const { return st_[j].used; }
-  bool active(int j) const { return st_[j].active; }
-  int iters(int j) const { return st_[j].iters; }
-  double relres(int j) const { return st_[j].relres; }
-  bool converged(int j) const { return st_[j].relres <= st_[j].tol; }
+  bool active(int j) const { return st_[j].m.active != 0; }
+  int iters(int j) const { return st_[j].m.iters; }
+  double relres(int j) const { return st_[j].m.relres; }
+  bool converged(int j) const { return st_[j].m.relres <= st_[j].m.tol; }
   bool any_active() const {
     for (const Slot& s : st_)
-      if (s.active) return true;
+      if (s.m.active) return true;
     return false;
   }
   int n_used() const {
@@ -235,6 +287,7 @@ class MinresBatch {
                 const std::vector<int>& max_iter) {
     const int k = B->nvec;
     PHX_REQUIRE(k == cap_, PHIST_INVALID_INPUT, "load_all needs %d columns", cap_);
+    drain();
     blas_put_value<T>(work_.get(), T(0));
     Mvec<T>* v = cur(V);
     copy_any<TS, T>(B, v);
@@ -246,6 +299,7 @@ class MinresBatch {
       start(j, (double)nrm[j], shifts[j], tol[j], max_iter[j]);
     }
     blas_scale_cols<T>(v, sc.data(), false);
+    if (dev_) upload(0, k);
   }
 
   // slot j from column c of B (x = 0)
@@ -264,6 +318,7 @@ class MinresBatch {
     T sc = nrm > T(0) ? T(1) / nrm : T(0);
     blas_scale_cols<T>(vj.get(), &sc, true);
     start(j, (double)nrm, shift, tol, max_iter);
+    if (dev_) upload(j, 1);
   }
 
   // solution of slot j into column c of X (any precision)
@@ -277,13 +332,105 @@ class MinresBatch {
   void store_all(Mvec<TD>* X) const {
     copy_any<T, TD>(cur(XS), X);
   }
-  void release(int j) { st_[j] = Slot(); }
+  void release(int j) {
+    st_[j] = Slot();
+    if (dev_) upload(j, 1);  // frozen on the device as well
+  }
 
   // one MINRES iteration of every active slot
   void iterate() {
+    if (dev_)
+      iterate_dev();
+    else
+      iterate_host();
+  }
+
+  // bring the host's view of the slot states up to date with every queued iteration
+  void drain() {
+    if (dev_ && pending_ >= 0) {
+      apply(pending_);
+      pending_ = -1;
+    }
+  }
+
+ private:
+  enum { V = 0, VOLD = 1, W = 2, D1 = 3, D2 = 4, XS = 5 };
+  struct Slot {
+    bool used = false;
+    T shift = T(0);
+    MinresState m{0, 0, 1, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0};
+  };
+  struct Blk {
+    Mvec<T>* p = nullptr;
+    ~Blk() {
+      try {
+        if (p) api::mvec_delete(p);
+      } catch (...) {
+      }
+    }
+  };
+  Mvec<T>* cur(int r) const { return blk_[role_[r]].p; }
+  MinresState* h_up() const { return h_buf_ + 2 * cap_; }
+  void start(int j, double beta1, T shift, double tol, int max_iter) {
+    Slot q;
+    q.used = true;
+    q.shift = shift;
+    q.m.beta1 = beta1;
+    q.m.phibar = beta1;
+    q.m.tol = tol;
+    q.m.max_iter = max_iter;
+    q.m.active = beta1 > 0.0 && max_iter > 0;
+    q.m.relres = beta1 > 0.0 ? 1.0 : 0.0;
+    st_[j] = q;
+  }
+  // slot states j0..j0+n-1 to the device; copies of the device state queued
+  // before this upload no longer speak for these slots
+  void upload(int j0, int n) {
+    if (uploads_) PHX_HIP(hipEventSynchronize(ev_[2]));  // the staging area is free again
+    for (int j = j0; j < j0 + n; j++) {
+      h_up()[j] = st_[j].m;
+      load_seq_[j] = seq_ + 1;
+    }
+    PHX_HIP(hipMemcpyAsync(d_st_ + j0, h_up() + j0, sizeof(MinresState) * n, hipMemcpyHostToDevice, rt().stream));
+    PHX_HIP(hipEventRecord(ev_[2], rt().stream));
+    uploads_++;
+  }
+  void apply(int b) {
+    PHX_HIP(hipEventSynchronize(ev_[b]));
+    const MinresState* src = h_buf_ + (size_t)b * cap_;
+    for (int j = 0; j < cap_; j++)
+      if (copy_seq_[b] >= load_seq_[j]) st_[j].m = src[j];
+  }
+
+  void iterate_dev() {
+    const int k = cap_;
+    std::vector<T> s(k);
+    for (int j = 0; j < k; j++) s[j] = st_[j].shift;
+    Mvec<T>* v = cur(V);
+    Mvec<T>* w = cur(W);
+    op_.apply_shifted_dev(s.data(), v, w, d_dots_);
+    if (C_) {
+      gemm_gram<T>(T(1), Q_, w, T(0), C_->get(), false);
+      gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
+    }
+    krylov_minres_step_dev<T>(v->nrows, k, work_->lda, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(),
+                              cur(D2)->dev(), cur(XS)->dev(), d_dots_, d_st_, d_work_);
+    std::swap(role_[V], role_[VOLD]);
+    std::swap(role_[D1], role_[D2]);
+    ++seq_;
+    const int b = (int)(seq_ & 1);
+    hipStream_t st = rt().stream;
+    PHX_HIP(hipMemcpyAsync(h_buf_ + (size_t)b * cap_, d_st_, sizeof(MinresState) * cap_, hipMemcpyDeviceToHost, st));
+    PHX_HIP(hipEventRecord(ev_[b], st));
+    copy_seq_[b] = seq_;
+    if (pending_ >= 0) apply(pending_);
+    pending_ = b;
+  }
+
+  void iterate_host() {
     const int k = cap_;
     std::vector<T> s(k), alpha(k), la(k), lb(k), bnext(k), dl(k), ep(k), ig(k), ta(k), ib(k);
-    for (int j = 0; j < k; j++) s[j] = st_[j].active ? st_[j].shift : T(0);
+    for (int j = 0; j < k; j++) s[j] = st_[j].m.active ? st_[j].shift : T(0);
     Mvec<T>* v = cur(V);
     Mvec<T>* w = cur(W);
     op_.apply_shifted(T(1), s.data(), v, T(0), w, alpha.data());
@@ -292,16 +439,16 @@ class MinresBatch {
       gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
     }
     for (int j = 0; j < k; j++) {
-      la[j] = st_[j].active ? alpha[j] : T(0);
-      lb[j] = st_[j].active && st_[j].iters > 0 ? (T)st_[j].beta : T(0);
+      la[j] = st_[j].m.active ? alpha[j] : T(0);
+      lb[j] = st_[j].m.active && st_[j].m.iters > 0 ? (T)st_[j].m.beta : T(0);
     }
     const int64_t ld = work_->lda;
     krylov_lanczos<T>(v->nrows, k, ld, v->dev(), cur(VOLD)->dev(), w->dev(), la.data(), lb.data(), bnext.data());
     for (int j = 0; j < k; j++) {
       dl[j] = ep[j] = ig[j] = ta[j] = ib[j] = T(0);
-      Slot& q = st_[j];
+      MinresState& q = st_[j].m;
       if (!q.active) continue;
-      // column `it` of the Lanczos tridiagonal: beta_k above, alpha_k on and
+      // column `iters` of the Lanczos tridiagonal: beta_k above, alpha_k on and
       // beta_{k+1} below the diagonal; two old rotations, one new one
       const double bk = q.iters > 0 ? q.beta : 0.0, ak = (double)alpha[j], bn = (double)bnext[j];
       const double eps_k = q.s2 * bk, dbar = q.c2 * bk;
@@ -309,7 +456,7 @@ class MinresBatch {
       const double gbar = -q.s1 * dbar + q.c1 * ak;
       const double gamma = std::hypot(gbar, bn);
       if (!(gamma > 0.0) || !std::isfinite(gamma)) {  // singular or broken down: stop this system
-        q.active = false;
+        q.active = 0;
         continue;
       }
       const double c = gbar / gamma, sn = bn / gamma;
@@ -327,43 +474,12 @@ class MinresBatch {
       q.beta = bn;
       q.iters++;
       q.relres = std::fabs(q.phibar) / q.beta1;
-      if (q.relres <= q.tol || bn == 0.0 || q.iters >= q.max_iter) q.active = false;
+      if (q.relres <= q.tol || bn == 0.0 || q.iters >= q.max_iter) q.active = 0;
     }
     krylov_minres_update<T>(v->nrows, k, ld, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(), cur(D2)->dev(),
                             cur(XS)->dev(), dl.data(), ep.data(), ig.data(), ta.data(), ib.data());
     std::swap(role_[V], role_[VOLD]);  // the next Lanczos vector was written over the old one
     std::swap(role_[D1], role_[D2]);   // the new direction was written over d_{k-2}
-  }
-
- private:
-  enum { V = 0, VOLD = 1, W = 2, D1 = 3, D2 = 4, XS = 5 };
-  struct Slot {
-    bool used = false, active = false;
-    double beta1 = 0, beta = 0, c1 = 1, s1 = 0, c2 = 1, s2 = 0, phibar = 0, relres = 1, tol = 0;
-    int iters = 0, max_iter = 0;
-    T shift = T(0);
-  };
-  struct Blk {
-    Mvec<T>* p = nullptr;
-    ~Blk() {
-      try {
-        if (p) api::mvec_delete(p);
-      } catch (...) {
-      }
-    }
-  };
-  Mvec<T>* cur(int r) const { return blk_[role_[r]].p; }
-  void start(int j, double beta1, T shift, double tol, int max_iter) {
-    Slot q;
-    q.used = true;
-    q.beta1 = beta1;
-    q.phibar = beta1;
-    q.shift = shift;
-    q.tol = tol;
-    q.max_iter = max_iter;
-    q.active = beta1 > 0.0 && max_iter > 0;
-    q.relres = beta1 > 0.0 ? 1.0 : 0.0;
-    st_[j] = q;
   }
 
   const Operator<T>& op_;
@@ -375,6 +491,16 @@ class MinresBatch {
   int role_[6] = {0, 1, 2, 3, 4, 5};
   std::vector<Slot> st_;
   std::unique_ptr<SdBox<T>> C_;
+  // device-resident scalars
+  bool dev_ = false;
+  MinresState* d_st_ = nullptr;
+  T* d_dots_ = nullptr;
+  T* d_work_ = nullptr;
+  MinresState* h_buf_ = nullptr;  // pinned: two lagged state copies, then the upload staging
+  hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};  // two lagged state copies, the last upload
+  int64_t seq_ = 0, copy_seq_[2] = {0, 0}, uploads_ = 0;
+  int pending_ = -1;
+  std::vector<int64_t> load_seq_;
 };
 
 template <typename T>
@@ -384,36 +510,62 @@ struct MinresOut {
   int unconverged = 0;
 };
 
+// Repeated solves of k systems with changing shifts, projectors and
+// right-hand sides (the correction equations of one eigensolver run): the
+// batch and, for a lower inner precision TI, the converted projection basis
+// are kept between solves.
+template <typename T, typename TI>
+class MinresSolver {
+ public:
+  MinresSolver(const Operator<TI>& op, const Map* map, int k) : mb_(op, map, k), map_(map) {}
+  MinresOut<TI> solve(const std::vector<T>& s, const Mvec<T>* Q, const Mvec<T>* B, Mvec<T>* X,
+                      const std::vector<double>& tol, int maxIter) {
+    const int k = B->nvec;
+    const Mvec<TI>* Qi = nullptr;
+    std::unique_ptr<View<TI>> qv;
+    if (Q && Q->nvec > 0) {
+      if constexpr (std::is_same<T, TI>::value) {
+        Qi = Q;
+      } else {
+        if (!q_.p || q_->nvec < Q->nvec) {
+          MvBox<TI> fresh(map_, Q->nvec);
+          std::swap(q_.p, fresh.p);
+        }
+        qv.reset(new View<TI>(q_.get(), 0, Q->nvec - 1));
+        copy_any<T, TI>(Q, qv->get());
+        Qi = qv->get();
+      }
+    }
+    mb_.set_projection(Qi);
+    std::vector<TI> si(s.begin(), s.end());
+    mb_.load_all(B, si, tol, std::vector<int>(k, maxIter));
+    while (mb_.any_active()) mb_.iterate();
+    mb_.drain();
+    mb_.store_all(X);
+    mb_.set_projection(nullptr);
+    MinresOut<TI> out;
+    out.iters.resize(k);
+    out.relres.resize(k);
+    for (int j = 0; j < k; j++) {
+      out.iters[j] = mb_.iters(j);
+      out.relres[j] = mb_.relres(j);
+      if (!mb_.converged(j)) out.unconverged++;
+    }
+    return out;
+  }
+
+ private:
+  MinresBatch<TI> mb_;
+  const Map* map_;
+  MvBox<TI> q_;
+};
+
 // k systems at once in precision TI (B and X in precision T, Q converted)
 template <typename T, typename TI>
 MinresOut<TI> minres_solve(const Operator<TI>& op, const std::vector<T>& s, const Mvec<T>* Q, const Mvec<T>* B,
                            Mvec<T>* X, const std::vector<double>& tol, int maxIter) {
-  const int k = B->nvec;
-  MvBox<TI> q;
-  const Mvec<TI>* Qi = nullptr;
-  if (Q && Q->nvec > 0) {
-    if constexpr (std::is_same<T, TI>::value) {
-      Qi = Q;
-    } else {
-      q.p = api::mvec_create<TI>(Q->map, Q->nvec);
-      copy_any<T, TI>(Q, q.get());
-      Qi = q.get();
-    }
-  }
-  MinresBatch<TI> mb(op, B->map, k, Qi);
-  std::vector<TI> si(s.begin(), s.end());
-  mb.load_all(B, si, tol, std::vector<int>(k, maxIter));
-  while (mb.any_active()) mb.iterate();
-  mb.store_all(X);
-  MinresOut<TI> out;
-  out.iters.resize(k);
-  out.relres.resize(k);
-  for (int j = 0; j < k; j++) {
-    out.iters[j] = mb.iters(j);
-    out.relres[j] = mb.relres(j);
-    if (!mb.converged(j)) out.unconverged++;
-  }
-  return out;
+  MinresSolver<T, TI> solver(op, B->map, B->nvec);
+  return solver.solve(s, Q, B, X, tol, maxIter);
 }
 
 // resumable block Jacobi-Davidson on a sparse matrix of this library

@@ -316,7 +316,12 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
     int lpr = 1;
     while (lpr < groups) lpr <<= 1;
     const int grid = VPL == 1 ? launch_lpr<T, 1>(a, lpr) : launch_lpr<T, V16>(a, lpr);
-    if (dots) {
+    if (dots && req.dots_dev) {
+      PHX_REQUIRE(cb == 0 && nc == nvec, PHIST_INVALID_INPUT, "device dots need one column block (%d columns)", nvec);
+      hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
+                         grid, 2 * nc, static_cast<T*>(req.dots_dev));
+      PHX_HIP(hipGetLastError());
+    } else if (dots) {
       std::vector<T> sums(2 * nc);
       reduce_partials_to_host(a.partials, grid, 2 * nc, sums.data());
       for (int c = 0; c < nc; c++) {

@@ -311,7 +311,7 @@ static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int
 }
 
 template <typename T>
-void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C) {
+void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C, bool to_host) {
   const int m = V->nvec, k = W->nvec;
   PHX_REQUIRE(V->nrows == W->nrows, PHIST_INVALID_INPUT, "V and W have different lengths");
   PHX_REQUIRE(C->nrows == m && C->ncols == k, PHIST_INVALID_INPUT, "C is %dx%d, V'W is %dx%d", C->nrows, C->ncols, m,
@@ -334,7 +334,7 @@ void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C)
         gram_block_mfma<T>(alpha, V->dev() + i0, V->lda, mi, W->dev() + j0, W->lda, kj, V->nrows, diag_self, beta,
                            cblk, ldc, cmir);
       }
-    sd_from_device(C);
+    if (to_host) sd_from_device(C);
     return;
   }
   const bool swap = k > m;
@@ -347,7 +347,7 @@ void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C)
     T* cblk = swap ? C->dev() + b0 : C->dev() + (int64_t)b0 * ldc;
     gram_block<T>(alpha, A->dev(), A->lda, na, B->dev() + b0, B->lda, nbc, A->nrows, swap, beta, cblk, ldc);
   }
-  sd_from_device(C);
+  if (to_host) sd_from_device(C);
 }
 
 // ---------------------------------------------------------------------------
@@ -863,7 +863,7 @@ void gemm_fused_mvsd_mvTmv(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta,
 }
 
 #define PHX_INST(T)                                                                                  \
-  template void gemm_gram<T>(T, const Mvec<T>*, const Mvec<T>*, T, SdMat<T>*);                       \
+  template void gemm_gram<T>(T, const Mvec<T>*, const Mvec<T>*, T, SdMat<T>*, bool);                       \
   template void gemm_times_sd<T>(T, const Mvec<T>*, const SdMat<T>*, T, Mvec<T>*);                   \
   template void gemm_times_sd_inplace<T>(Mvec<T>*, const SdMat<T>*);                                 \
   template void gemm_times_sd_add_times_sd<T>(const Mvec<T>*, const SdMat<T>*, Mvec<T>*, const SdMat<T>*); \
