@@ -41,13 +41,13 @@ def spmv_bytes(nnz, n, nvec, sval=8, sx=8, beta_nonzero=False):
     return nnz * (sval + 4) + 4 * n + n * nvec * sx * (2 + (1 if beta_nonzero else 0))
 
 
-def read_traffic(path):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None.
-    The file holds lines `traffic_bytes_per_launch: <float>`."""
+def read_traffic(path, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per the unit `key` names) from a committed rocprofv3 --pmc summary,
+    or None.  The file holds lines `<key>: <float>`."""
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -360,7 +360,9 @@ def carp_bench(be, args, W=2048, L=2048):
                       "device_ms_per_sweep": round(dev_ms, 4)},
            "roofline": {"bound": "hbm", "achieved": round(nbytes / (dev_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(nbytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                        "traffic": None},
+                        "traffic": read_traffic(os.path.join(ROOT, "profiles",
+                                                             f"carp_sweep_graphene{W}_nvec{nvec}_traffic.txt"),
+                                                "traffic_bytes_per_sweep")},
            "cpu_baseline": cpu}
     W_.delete()
     X.delete()
