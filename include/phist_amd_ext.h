@@ -16,13 +16,17 @@ extern "C" {
 #endif
 
 /* SELL-C-sigma parameters for matrices created after the call
- * (C a power of two, 0 = default 32; sigma 0 = default 256). */
+ * (C a power of two, 0 = default 32; sigma 0 = chosen per matrix: 1, which
+ * keeps the rows in their own order, when that pads the chunks by at most
+ * 25 %, else 256). */
 void phist_amd_sparseMat_set_format(int C, int sigma, int* iflag);
 /* upper bound for the persistent grid of the streaming kernels (0 = one
  * resident wave of workgroups on every CU) */
 void phist_amd_set_grid_blocks(int nblocks, int* iflag);
-/* workgroups per CU of the SpMV's persistent grid (0 = as many as fit; the
- * default 2 keeps the x rows gathered by a wide band L2-resident) */
+/* workgroups per CU of the SpMV's persistent grid (0 = as many as fit;
+ * n < 0 = the built-in table, the default: 2 for rows of >= 24 nonzeros and
+ * blocks of >= 4 columns, whose gathers then keep a wide band's x rows
+ * L2-resident, 3 for such rows and narrower blocks, 6 for shorter rows) */
 void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
 
 /* the library's HIP stream (hipStream_t), device synchronisation, and an

@@ -87,7 +87,7 @@ struct Runtime {
   int grid_blocks = 0;
   // workgroups per CU for the SpMV: the x gathers of a +-band thrash L2 when
   // too many rows are in flight, two workgroups per CU measured best
-  int spmv_blocks_per_cu = 2;
+  int spmv_blocks_per_cu = -1;  // < 0: spmv_table_blocks_per_cu
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
   // workgroups per CU of the Gram kernels (each wave leaves one partial tile,

@@ -401,7 +401,10 @@ class JadaRun {
       // no rank truncation here: near convergence t is tiny, and a rank
       // test would replace it by noise (the reference passes no flags)
       api::orthog<T>(Vful.get(), Vv.get(), R1.get(), R2.get(), 3, &rank, 0);
-      A.apply_shifted(T(1), nullptr, Vv.get(), T(0), AVv.get(), nullptr);
+      // the SpMV gathers rows of its input: take them from the narrow block t
+      // (row stride bs) rather than from the wide basis (row stride maxBas)
+      blas_copy<T>(Vv.get(), t.get());
+      A.apply_shifted(T(1), nullptr, t.get(), T(0), AVv.get(), nullptr);
       View<T> Vall(V_.get(), 0, nV + k - 1);
       SdBox<T> Hn(nV + k, k);
       gemm_gram<T>(T(1), Vall.get(), AVv.get(), T(0), Hn.get());

@@ -479,9 +479,8 @@ void phist_amd_set_grid_blocks(int nblocks, int* iflag) {
 }
 void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag) {
   guard(iflag, __func__, [&] {
-    PHX_REQUIRE(n >= 0, PHIST_INVALID_INPUT, "negative workgroup count");
     ensure_init();
-    rt().spmv_blocks_per_cu = n;
+    rt().spmv_blocks_per_cu = n < 0 ? -1 : n;
     *iflag = 0;
   });
 }

@@ -221,11 +221,15 @@ template <typename T>
 class MinresBatch {
  public:
   MinresBatch(const Operator<T>& op, const Map* map, int cap, const Mvec<T>* Q = nullptr)
-      : op_(op), map_(map), cap_(cap), Q_(Q), work_(map, 6 * cap), st_(cap), load_seq_(cap, 0) {
+      : op_(op), map_(map), cap_(cap), Q_(Q), st_(cap), load_seq_(cap, 0) {
     PHX_REQUIRE(cap >= 1 && cap <= 64, PHIST_INVALID_INPUT, "MINRES batch of %d systems", cap);
-    for (int b = 0; b < 6; b++) blk_[b].p = nullptr;
-    for (int b = 0; b < 6; b++) api::mvec_view_block(work_.get(), &blk_[b].p, b * cap, (b + 1) * cap - 1);
-    blas_put_value<T>(work_.get(), T(0));
+    // six separate blocks of `cap` columns (row stride cap each): the SpMV
+    // gathers rows of V, and a V whose rows are interleaved with the other
+    // blocks would waste most of every cache line it touches
+    for (int b = 0; b < 6; b++) {
+      blk_[b].p = api::mvec_create<T>(map, cap);
+      blas_put_value<T>(blk_[b].p, T(0));
+    }
     if (Q_ && Q_->nvec > 0) C_.reset(new SdBox<T>(Q_->nvec, cap));
     dev_ = op_.A != nullptr && rt().minres_device;
     if (dev_) {
@@ -288,7 +292,7 @@ class MinresBatch {
     const int k = B->nvec;
     PHX_REQUIRE(k == cap_, PHIST_INVALID_INPUT, "load_all needs %d columns", cap_);
     drain();
-    blas_put_value<T>(work_.get(), T(0));
+    for (int b = 0; b < 6; b++) blas_put_value<T>(blk_[b].p, T(0));
     Mvec<T>* v = cur(V);
     copy_any<TS, T>(B, v);
     std::vector<T> nrm(k), sc(k);
@@ -413,7 +417,7 @@ class MinresBatch {
       gemm_gram<T>(T(1), Q_, w, T(0), C_->get(), false);
       gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
     }
-    krylov_minres_step_dev<T>(v->nrows, k, work_->lda, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(),
+    krylov_minres_step_dev<T>(v->nrows, k, cap_, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(),
                               cur(D2)->dev(), cur(XS)->dev(), d_dots_, d_st_, d_work_);
     std::swap(role_[V], role_[VOLD]);
     std::swap(role_[D1], role_[D2]);
@@ -442,7 +446,7 @@ class MinresBatch {
       la[j] = st_[j].m.active ? alpha[j] : T(0);
       lb[j] = st_[j].m.active && st_[j].m.iters > 0 ? (T)st_[j].m.beta : T(0);
     }
-    const int64_t ld = work_->lda;
+    const int64_t ld = cap_;
     krylov_lanczos<T>(v->nrows, k, ld, v->dev(), cur(VOLD)->dev(), w->dev(), la.data(), lb.data(), bnext.data());
     for (int j = 0; j < k; j++) {
       dl[j] = ep[j] = ig[j] = ta[j] = ib[j] = T(0);
@@ -486,8 +490,7 @@ class MinresBatch {
   const Map* map_;
   int cap_;
   const Mvec<T>* Q_;
-  MvBox<T> work_;
-  Blk blk_[6];  // views of work_ (destroyed before work_)
+  Blk blk_[6];  // V, VOLD, W, D1, D2, XS (roles rotate)
   int role_[6] = {0, 1, 2, 3, 4, 5};
   std::vector<Slot> st_;
   std::unique_ptr<SdBox<T>> C_;

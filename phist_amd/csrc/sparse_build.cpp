@@ -89,7 +89,7 @@ template <typename T>
 void sell_build(SparseMat<T>* A, int C, int sigma) {
   Runtime& r = rt();
   if (C <= 0) C = r.sell_C > 0 ? r.sell_C : 32;
-  if (sigma <= 0) sigma = r.sell_sigma > 0 ? r.sell_sigma : 256;
+  if (sigma <= 0) sigma = r.sell_sigma;
   PHX_REQUIRE((C & (C - 1)) == 0 && C <= 1024, PHIST_INVALID_INPUT, "SELL chunk height %d is not a power of two", C);
   const int64_t n = A->nrows;
   std::vector<int32_t> len(n);
@@ -97,6 +97,18 @@ void sell_build(SparseMat<T>* A, int C, int sigma) {
   for (int64_t i = 0; i < n; i++) {
     len[i] = (int32_t)(A->crs_ptr[i + 1] - A->crs_ptr[i]);
     if (i && len[i] != len[0]) uniform = false;
+  }
+  if (sigma <= 0) {
+    // rows in their own order keep the locality of the matrix's numbering
+    // (neighbouring rows gather neighbouring x rows); sort by length only
+    // when that order pads the chunks by more than a quarter
+    int64_t slots = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += C) {
+      int32_t w = 0;
+      for (int64_t i = c0; i < std::min<int64_t>(n, c0 + C); i++) w = std::max(w, len[i]);
+      slots += (int64_t)w * C;
+    }
+    sigma = (double)slots <= 1.25 * (double)std::max<int64_t>(1, A->nnz) ? 1 : 256;
   }
   const int64_t nchunks = (n + C - 1) / C;
   const int64_t nslots = nchunks * C;

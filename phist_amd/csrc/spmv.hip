@@ -42,9 +42,21 @@ struct SpmvArgs {
   int ncols;  // block columns handled by this launch
   T alpha, beta, gamma, delta;
   int flags;
+  int blocks_per_cu;  // host side: persistent grid size per CU (0 = as many as fit)
   T* __restrict__ partials;  // [gridDim.x][2][ncols]
   T shift[64 * 4];
 };
+
+// Workgroups per CU by row length and block width, from a measured sweep
+// (1..8 per CU on MI355X; FP64, nvec 1..16; banded 32-nnz rows, XXZ chain,
+// 3D Laplacian, honeycomb sheets): long rows gathered from a wide band run
+// best with few workgroups (the gathered x rows stay in L2), short rows need
+// more waves in flight to cover the gather latency.
+inline int spmv_table_blocks_per_cu(double nnz_per_row, int nvec) {
+  if (nnz_per_row >= 24.0) return nvec >= 4 ? 2 : 3;
+  if (nnz_per_row >= 16.0) return 4;
+  return 6;
+}
 
 enum : int { F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16 };
 
@@ -234,7 +246,7 @@ int launch_one(SpmvArgs<T>& a) {
   auto kern = k_sell_spmv<T, VPL, LPR>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   a.ntiles = (a.nslots + RPB - 1) / RPB;
-  const int grid = persistent_blocks(kern, 0, a.ntiles, rt().spmv_blocks_per_cu);
+  const int grid = persistent_blocks(kern, 0, a.ntiles, a.blocks_per_cu);
   if (a.flags & (F_DWW | F_DVW))
     a.partials = (T*)device_work(sizeof(T) * ((size_t)grid * 2 * a.ncols + 2 * a.ncols));
   ktimer_begin();
@@ -303,6 +315,9 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   a.delta = delta;
   a.flags = (req.has_shift ? F_SHIFT : 0) | (req.has_beta ? F_BETA : 0) | (req.dot_ww ? F_DWW : 0) |
             (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0);
+  a.blocks_per_cu = rt().spmv_blocks_per_cu >= 0
+                        ? rt().spmv_blocks_per_cu
+                        : spmv_table_blocks_per_cu(A->nrows ? (double)A->nnz / (double)A->nrows : 0.0, nvec);
   const bool dots = req.dot_ww || req.dot_vw;
 
   for (int cb = 0; cb < nvec; cb += maxcols) {
