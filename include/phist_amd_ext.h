@@ -49,6 +49,12 @@ void phist_amd_kiss_reset(int* iflag);
 void phist_amd_kiss_skip(uint64_t n, int* iflag);
 void phist_amd_kiss_draw(int64_t n, double* out, int* iflag);
 
+/* the host's symmetric eigensolvers (no device work): a (n x n, column-major,
+ * lda) is overwritten by the eigenvectors, w gets the eigenvalues ascending;
+ * method 0 = cyclic Jacobi (SVQB), 1 = Householder + implicit QL (the
+ * eigensolver's Rayleigh-Ritz problems) */
+void phist_amd_dense_sym_eig(double* a, int n, int lda, double* w, int method, int* iflag);
+
 /* host copies of a matrix's index data.  get_crs: builtin-order CRS with
  * 1-based row_offset (nrows+1) and col_idx (nnz).  get_sell: C, sigma,
  * chunk count and padded slot count; arrays may be NULL to query sizes only

@@ -33,6 +33,7 @@ namespace phx {
 
 namespace dense {
 void sym_eig(double* a, int n, int lda, double* w);
+void sym_eig_tridiag(double* a, int n, int lda, double* w);
 }
 
 namespace {
@@ -263,7 +264,7 @@ class JadaRun {
     for (int j = 0; j < m; j++)
       for (int i = 0; i < m; i++) hb[(size_t)j * m + i] = (double)H(nConv + i, nConv + j);
     std::vector<double> w(m);
-    dense::sym_eig(hb.data(), m, m, w.data());
+    dense::sym_eig_tridiag(hb.data(), m, m, w.data());
     const std::vector<int> ord = ritz_order<T>(w, which);
     qh.assign((size_t)m * m, 0.0);
     for (int j = 0; j < m; j++) {

@@ -166,7 +166,7 @@ void kiss_draw(int64_t n, double* out) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (!g_tables) build_tables();
   const Kiss start = g_state;
-  const int64_t seg = 1 << 20;
+  const int64_t seg = 1 << 18;
   const int64_t nseg = (n + seg - 1) / seg;
 #pragma omp parallel for schedule(dynamic, 1) if (nseg > 1)
   for (int64_t k = 0; k < nseg; k++) {
