@@ -95,7 +95,8 @@ struct Runtime {
   int gram_blocks_per_cu = 8;
   // row-mixing products (V*C, in-place V*M) with k, m >= 8 run on the matrix cores
   bool mix_mfma = true;
-  bool minres_device = true;  // MINRES scalars on the device for this library's matrices
+  bool minres_device = true;
+  bool gram_rows = true;  // narrow unaligned Grams: several columns per lane (k_gram_rows)  // MINRES scalars on the device for this library's matrices
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;

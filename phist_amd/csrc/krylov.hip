@@ -38,6 +38,13 @@ struct KrylovArgs {
   // coefficients: from cdev (device, coefficient q of column c at cdev[q*ncols + c]) when it is set,
   // else from the kernel arguments c0..c4
   const T* cdev;
+  // optional projection fused into k_lanczos: w -= Q*C (Q nrows x nq, row stride ldq; C nq x ncols,
+  // column-major with leading dimension ldcm, on the device)
+  const T* q;
+  int64_t ldq;
+  int nq;
+  const T* cmat;
+  int ldcm;
   T c0[kKrylovMaxCols], c1[kKrylovMaxCols], c2[kKrylovMaxCols], c3[kKrylovMaxCols], c4[kKrylovMaxCols];
 };
 
@@ -62,6 +69,18 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = lane % LPR, rw = lane / LPR, c0 = q * VPL;
   const bool colok = c0 < p.ncols;
+  // projection coefficients C(j, c0+e) in LDS, row-interleaved: sc[j*VPL*LPR + c]
+  extern __shared__ char lanczos_lds[];
+  T* sc = reinterpret_cast<T*>(lanczos_lds);
+  const int nq = p.nq;
+  if (nq > 0) {
+    const int width = VPL * LPR;
+    for (int e = threadIdx.x; e < nq * width; e += kBlock) {
+      const int j = e / width, c = e - j * width;
+      sc[e] = c < p.ncols ? p.cmat[(int64_t)c * p.ldcm + j] : T(0);
+    }
+    __syncthreads();
+  }
   T a[VPL], b[VPL], acc[VPL];
 #pragma unroll
   for (int e = 0; e < VPL; e++) {
@@ -77,6 +96,15 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
     T vv[VPL], ww[VPL];
     vload<T, VPL>(p.v + o, vv);
     vload<T, VPL>(p.w + o, ww);
+    if (nq > 0) {  // w(i,:) -= Q(i,:) * C
+      const T* qi = p.q + i * p.ldq;
+      const T* cj = sc + c0;
+      for (int j = 0; j < nq; j++, cj += VPL * LPR) {
+        const T qv = qi[j];
+#pragma unroll
+        for (int e = 0; e < VPL; e++) ww[e] = fma(-qv, cj[e], ww[e]);
+      }
+    }
 #pragma unroll
     for (int e = 0; e < VPL; e++) ww[e] = fma(-a[e], vv[e], ww[e]);
     if (p.flags & KF_VOLD) {
@@ -233,9 +261,10 @@ int launch_lpr(bool minres, KrylovArgs<T>& p) {
     grid = persistent_blocks(k_minres<T, VPL, LPR>, 0, p.ntiles);
     hipLaunchKernelGGL((k_minres<T, VPL, LPR>), dim3(grid), dim3(kBlock), 0, rt().stream, p);
   } else {
-    grid = persistent_blocks(k_lanczos<T, VPL, LPR>, 0, p.ntiles);
+    const size_t lds = p.nq > 0 ? sizeof(T) * (size_t)p.nq * VPL * LPR : 0;
+    grid = persistent_blocks(k_lanczos<T, VPL, LPR>, lds, p.ntiles);
     p.partials = (T*)device_work(sizeof(T) * ((size_t)grid * p.ncols + p.ncols));
-    hipLaunchKernelGGL((k_lanczos<T, VPL, LPR>), dim3(grid), dim3(kBlock), 0, rt().stream, p);
+    hipLaunchKernelGGL((k_lanczos<T, VPL, LPR>), dim3(grid), dim3(kBlock), lds, rt().stream, p);
   }
   PHX_HIP(hipGetLastError());
   return grid;
@@ -287,6 +316,11 @@ void krylov_lanczos(int64_t nrows, int ncols, int64_t ld, const T* v, const T* v
   p.w = w;
   p.flags = 0;
   p.cdev = nullptr;
+  p.q = nullptr;
+  p.ldq = 0;
+  p.nq = 0;
+  p.cmat = nullptr;
+  p.ldcm = 0;
   PHX_REQUIRE(ncols <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns too wide", ncols);
   for (int c = 0; c < ncols; c++) {
     p.c0[c] = a[c];
@@ -316,6 +350,11 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
   p.x = x;
   p.flags = 0;
   p.cdev = nullptr;
+  p.q = nullptr;
+  p.ldq = 0;
+  p.nq = 0;
+  p.cmat = nullptr;
+  p.ldcm = 0;
   PHX_REQUIRE(ncols <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns too wide", ncols);
   for (int c = 0; c < ncols; c++) {
     p.c0[c] = delta[c];
@@ -336,7 +375,8 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
 // coefficients) plus k squared norms.
 template <typename T>
 void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
-                            const T* dots, MinresState* st, T* work) {
+                            const T* dots, MinresState* st, T* work, const T* Q, int64_t ldq, int nq, const T* C,
+                            int ldc) {
   PHX_REQUIRE(k >= 1 && k <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns", k);
   if (nrows == 0) return;  // every column starts frozen (zero right-hand side)
   hipStream_t s = rt().stream;
@@ -357,7 +397,14 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   p.x = x;
   p.flags = KF_VOLD;
   p.cdev = ab;
+  p.q = Q;
+  p.ldq = ldq;
+  p.nq = Q ? nq : 0;
+  p.cmat = C;
+  p.ldcm = ldc;
   const int grid = launch(false, p, {v, w, vold});
+  p.q = nullptr;
+  p.nq = 0;
   hipLaunchKernelGGL(k_reduce_partials<T>, dim3((k + 15) / 16), dim3(kBlock), 0, s, p.partials, grid, k, nrm2);
   PHX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_minres_rot<T>, dim3(1), dim3(64), 0, s, dots, nrm2, st, k, coef);
@@ -389,9 +436,11 @@ template void krylov_minres_update<float>(int64_t, int, int64_t, const float*, f
                                           float*, float*, const float*, const float*, const float*, const float*,
                                           const float*);
 template void krylov_minres_step_dev<double>(int64_t, int, int64_t, const double*, double*, double*, const double*,
-                                             double*, double*, const double*, MinresState*, double*);
+                                             double*, double*, const double*, MinresState*, double*, const double*,
+                                             int64_t, int, const double*, int);
 template void krylov_minres_step_dev<float>(int64_t, int, int64_t, const float*, float*, float*, const float*, float*,
-                                            float*, const float*, MinresState*, float*);
+                                            float*, const float*, MinresState*, float*, const float*, int64_t, int,
+                                            const float*, int);
 template void convert_block<double, float>(const double*, int64_t, float*, int64_t, int64_t, int);
 template void convert_block<float, double>(const float*, int64_t, double*, int64_t, int64_t, int);
 

@@ -53,6 +53,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_GRAM_BLOCKS_PER_CU")) r.gram_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA")) r.mix_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_DEVICE")) r.minres_device = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_GRAM_ROWS")) r.gram_rows = atoi(s) != 0;
   kiss_reset();
   r.initialized = true;
 }

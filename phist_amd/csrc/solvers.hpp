@@ -215,7 +215,10 @@ void copy_any(const Mvec<TS>* X, Mvec<TD>* Y) {
 // ---------------------------------------------------------------------------
 template <typename T>
 void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
-                            const T* dots, MinresState* st, T* work);
+                            const T* dots, MinresState* st, T* work, const T* Q = nullptr, int64_t ldq = 0,
+                            int nq = 0, const T* C = nullptr, int ldc = 0);
+
+constexpr int kFusedProjCols = 64;  // projection bases up to this width are applied inside k_lanczos
 
 template <typename T>
 class MinresBatch {
@@ -413,12 +416,15 @@ class MinresBatch {
     Mvec<T>* v = cur(V);
     Mvec<T>* w = cur(W);
     op_.apply_shifted_dev(s.data(), v, w, d_dots_);
+    const bool fuse = C_ && Q_->nvec <= kFusedProjCols;  // w -= Q*C inside the Lanczos update
     if (C_) {
       gemm_gram<T>(T(1), Q_, w, T(0), C_->get(), false);
-      gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
+      if (!fuse) gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
     }
     krylov_minres_step_dev<T>(v->nrows, k, cap_, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(),
-                              cur(D2)->dev(), cur(XS)->dev(), d_dots_, d_st_, d_work_);
+                              cur(D2)->dev(), cur(XS)->dev(), d_dots_, d_st_, d_work_, fuse ? Q_->dev() : nullptr,
+                              fuse ? Q_->lda : 0, fuse ? Q_->nvec : 0, fuse ? C_->get()->dev() : nullptr,
+                              fuse ? C_->get()->root->ld : 0);
     std::swap(role_[V], role_[VOLD]);
     std::swap(role_[D1], role_[D2]);
     ++seq_;

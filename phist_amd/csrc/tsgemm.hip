@@ -90,6 +90,47 @@ __global__ void __launch_bounds__(kBlock) k_gram(const GramArgs<T> g) {
   });
 }
 
+// Gram a'b for a narrow b (nb <= NB) and na <= LPR*KA without vector loads
+// (rows not 16-byte aligned): a row per LPR lanes, lane q holding columns
+// q*KA .. q*KA+KA-1 of a and the whole row of b, so a wave covers 64/LPR
+// rows per step instead of one or two.
+template <typename T, int LPR, int KA, int NB>
+__global__ void __launch_bounds__(kBlock) k_gram_rows(const GramArgs<T> g) {
+  constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = lane % LPR, rw = lane / LPR, a0 = q * KA;
+  T acc[KA][NB];
+#pragma unroll
+  for (int e = 0; e < KA; e++)
+#pragma unroll
+    for (int c = 0; c < NB; c++) acc[e][c] = T(0);
+  const TileWalk tw = xcd_tiles(g.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t i = t * RPB + wv * RPW + rw;
+    if (i >= g.nrows) continue;
+    T av[KA], bv[NB];
+#pragma unroll
+    for (int e = 0; e < KA; e++) av[e] = a0 + e < g.na ? g.a[i * g.lda + a0 + e] : T(0);
+#pragma unroll
+    for (int c = 0; c < NB; c++) bv[c] = c < g.nb ? g.b[i * g.ldb + c] : T(0);
+#pragma unroll
+    for (int e = 0; e < KA; e++)
+#pragma unroll
+      for (int c = 0; c < NB; c++) acc[e][c] = fma(av[e], bv[c], acc[e][c]);
+  }
+  extern __shared__ char gram_lds[];
+  const int nout = g.na * g.nb;
+  block_tile_sum(reinterpret_cast<T*>(gram_lds), nout, g.partials + (int64_t)blockIdx.x * nout, [&](auto add) {
+#pragma unroll
+    for (int e = 0; e < KA; e++)
+#pragma unroll
+      for (int c = 0; c < NB; c++) {
+        const T s = wave_sum_strided<LPR>(acc[e][c]);
+        if (lane < LPR && a0 + e < g.na && c < g.nb) add(c * g.na + a0 + e, s);
+      }
+  });
+}
+
 // ---- MFMA Gram (both operands at least 16 columns) ----
 // v_mfma_{f64,f32}_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // (A 16x4, B 4x16) and holds four results of the 16x16 product; the result
@@ -209,6 +250,29 @@ int gram_launch_one(GramArgs<T>& g) {
   return grid;
 }
 
+template <typename T, int LPR, int KA, int NB>
+int gram_rows_launch(GramArgs<T>& g) {
+  auto kern = k_gram_rows<T, LPR, KA, NB>;
+  constexpr int RPB = (kBlock / 64) * (64 / LPR);
+  g.ntiles = (g.nrows + RPB - 1) / RPB;
+  const size_t lds = sizeof(T) * g.na * g.nb;
+  const int grid = persistent_blocks(kern, lds, g.ntiles, rt().gram_blocks_per_cu);
+  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * g.na * g.nb);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, g);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+  return grid;
+}
+
+// narrow b (nb <= 8), na <= 32, unaligned rows
+template <typename T>
+int gram_rows(GramArgs<T>& g) {
+  if (g.na <= 8) return g.nb <= 4 ? gram_rows_launch<T, 2, 4, 4>(g) : gram_rows_launch<T, 2, 4, 8>(g);
+  if (g.na <= 16) return g.nb <= 4 ? gram_rows_launch<T, 4, 4, 4>(g) : gram_rows_launch<T, 4, 4, 8>(g);
+  return g.nb <= 4 ? gram_rows_launch<T, 4, 8, 4>(g) : gram_rows_launch<T, 4, 8, 8>(g);
+}
+
 template <typename T, int VPL, int KB>
 int gram_lpr(GramArgs<T>& g, int lpr) {
   switch (lpr) {
@@ -295,7 +359,9 @@ static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int
     int lpr = 1;
     while (lpr < groups) lpr <<= 1;
     int nparts;  // partial tiles written (one per workgroup)
-    if (nb <= 8) {
+    if (VPL == 1 && nb <= 8 && g.na <= 32 && rt().gram_rows) {
+      nparts = gram_rows<T>(g);
+    } else if (nb <= 8) {
       nparts = VPL == 1 ? gram_lpr<T, 1, 8>(g, lpr) : gram_lpr<T, V16, 8>(g, lpr);
     } else {
       nparts = VPL == 1 ? gram_lpr<T, 1, 32>(g, lpr) : gram_lpr<T, V16, 32>(g, lpr);

@@ -120,7 +120,8 @@ def test_random_is_kiss_bitexact(be64):
 
 
 @pytest.mark.parametrize("m,k", [(1, 1), (4, 4), (8, 8), (3, 5), (16, 16), (32, 32), (32, 4), (2, 40), (64, 8),
-                                 (7, 7), (130, 3), (16, 24), (17, 33), (40, 40), (64, 48), (20, 16)])
+                                 (7, 7), (130, 3), (16, 24), (17, 33), (40, 40), (64, 48), (20, 16), (19, 4),
+                                 (31, 7), (33, 5), (9, 8), (1, 3), (15, 2)])
 @pytest.mark.parametrize("beta", [0.0, -0.5])
 def test_gram(be, m, k, beta):
     n = 5003
@@ -155,6 +156,27 @@ def test_gram_self_mfma(be, k):
     got = C.get().astype(float)
     assert np.all(np.abs(got - want) <= bound)
     for o in (v, C):
+        o.delete()
+
+
+@pytest.mark.parametrize("j0,j1,k", [(2, 20, 4), (0, 18, 4), (1, 5, 3), (3, 22, 8)])
+def test_gram_narrow_views(be, j0, j1, k):
+    """Q(:, j0..j1)' W with Q a column view of an odd-width block (rows not 16-byte aligned), as in the
+    projected correction equations"""
+    n = 6007
+    rng = np.random.default_rng(j0 * 31 + j1)
+    X, Wh = rng.uniform(-1, 1, (n, 23)), rng.uniform(-1, 1, (n, k))
+    mp = be.map(n)
+    x, w = be.mvec(mp, 23, X), be.mvec(mp, k, Wh)
+    xv = x.view(j0, j1)
+    m = j1 - j0 + 1
+    C = be.sdmat(m, k)
+    be.gram(1.0, xv, w, 0.0, C)
+    Vt, Wt = X[:, j0:j1 + 1].astype(be.dtype).astype(float), Wh.astype(be.dtype).astype(float)
+    want = ol.gram(1.0, Vt, Wt, 0.0, np.zeros((m, k)))
+    bound = 8 * RTOL[be.dtype.type] * (np.abs(Vt).T @ np.abs(Wt))
+    assert np.all(np.abs(C.get().astype(float) - want) <= bound)
+    for o in (xv, x, w, C):
         o.delete()
 
 
