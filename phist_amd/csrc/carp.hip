@@ -6,7 +6,9 @@
 // colouring of the row graph, a row's support being its columns plus its own
 // index, which the shift term updates); the rows of a colour then update in
 // parallel, one thread per (row, column of X), and a sweep runs the colours
-// forward and then backward.  The per-row arithmetic is that of builtin's
+// forward and then backward.  Each colour's rows keep their entries in a
+// chunked column-major copy (coalesced reads across the rows of a wave).
+// The per-row arithmetic is that of builtin's
 // kacz_sub_def.h:27-139 (shifted row product, row norm corrected for the
 // shift, scaled projection with relaxation omega_j), with fused multiply-adds
 // in a fixed place so that a sequential sweep in colour order reproduces it
@@ -18,16 +20,24 @@
 
 namespace phx {
 
+// Rows of one colour in chunks of kCarpChunk (in colour order), each chunk's
+// entries stored column-major (entry k of chunk row rr at chunk_ptr + k*kCarpChunk
+// + rr, padded to the chunk's longest row), so the rows a wave updates at once
+// read their matrix entries in coalesced segments.
+constexpr int kCarpChunk = 32;
+
 template <typename T>
 struct CarpWork {
   int64_t n = 0;
   int nshifts = 0;
   int ncolors = 0;
-  std::vector<int64_t> color_ptr;  // rows of colour c: rows[color_ptr[c] .. color_ptr[c+1])
+  std::vector<int64_t> color_ptr;    // rows of colour c: rows[color_ptr[c] .. color_ptr[c+1])
+  std::vector<int64_t> color_chunk;  // first chunk of colour c
   std::vector<int32_t> h_rows;
-  int32_t* rows = nullptr;     // device
-  int64_t* row_ptr = nullptr;  // device CRS (0-based)
-  int32_t* col = nullptr;
+  int32_t* rows = nullptr;       // device: rows in colour order
+  int32_t* rlen = nullptr;       // device: their lengths
+  int64_t* chunk_ptr = nullptr;  // device: first slot of every chunk
+  int32_t* col = nullptr;        // device: chunked entries
   T* val = nullptr;
 };
 
@@ -40,14 +50,16 @@ struct CarpCoef {
 
 template <typename T, bool RC>
 __global__ void __launch_bounds__(kBlock)
-    k_kacz(const int32_t* __restrict__ rows, int64_t nr, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
-           const T* __restrict__ va, int nvec, const CarpCoef<T> coef, const T* __restrict__ b, int64_t ldb, T* xr,
-           T* xi, int64_t ldx) {
+    k_kacz(const int32_t* __restrict__ rows, const int32_t* __restrict__ rlen, const int64_t* __restrict__ cptr,
+           int64_t nr, const int32_t* __restrict__ ci, const T* __restrict__ va, int nvec, const CarpCoef<T> coef,
+           const T* __restrict__ b, int64_t ldb, T* xr, T* xi, int64_t ldx) {
   const int64_t total = nr * nvec;
   for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
     const int64_t r = e / nvec;
     const int c = (int)(e - r * nvec);
     const int64_t i = rows[r];
+    const int len = rlen[r];
+    const int64_t k0 = cptr[r / kCarpChunk] + (r % kCarpChunk);
     const T s_r = coef.sr[c], s_i = RC ? coef.si[c] : T(0), w = coef.w[c];
     const T xri = xr[i * ldx + c];
     T tr, ti = T(0);
@@ -59,8 +71,8 @@ __global__ void __launch_bounds__(kBlock)
       tr = -s_r * xri;
     }
     T d = T(0), nrm = T(0);
-    const int64_t k0 = rp[i], k1 = rp[i + 1];
-    for (int64_t k = k0; k < k1; k++) {
+    for (int q = 0; q < len; q++) {
+      const int64_t k = k0 + (int64_t)q * kCarpChunk;
       const int64_t j = ci[k];
       const T a = va[k];
       tr = fma(a, xr[j * ldx + c], tr);
@@ -82,7 +94,8 @@ __global__ void __launch_bounds__(kBlock)
     } else {
       xr[i * ldx + c] = fma(tr, s_r, xri);
     }
-    for (int64_t k = k0; k < k1; k++) {
+    for (int q = 0; q < len; q++) {
+      const int64_t k = k0 + (int64_t)q * kCarpChunk;
       const int64_t j = ci[k];
       const T a = va[k];
       xr[j * ldx + c] = fma(-tr, a, xr[j * ldx + c]);
@@ -148,19 +161,53 @@ void* carp_setup(const SparseMat<T>* A, int nshifts) {
   W->h_rows.resize(W->n);
   std::vector<int64_t> fill(W->color_ptr.begin(), W->color_ptr.end() - 1);
   for (int64_t i = 0; i < W->n; i++) W->h_rows[fill[color[i]]++] = (int32_t)i;
-  hipStream_t st = rt().stream;
-  const int64_t nnz = A->nnz;
-  PHX_HIP(hipMalloc(&W->rows, sizeof(int32_t) * std::max<int64_t>(1, W->n)));
-  PHX_HIP(hipMalloc(&W->row_ptr, sizeof(int64_t) * (W->n + 1)));
-  PHX_HIP(hipMalloc(&W->col, sizeof(int32_t) * std::max<int64_t>(1, nnz)));
-  PHX_HIP(hipMalloc(&W->val, sizeof(T) * std::max<int64_t>(1, nnz)));
-  if (W->n) PHX_HIP(hipMemcpyAsync(W->rows, W->h_rows.data(), sizeof(int32_t) * W->n, hipMemcpyHostToDevice, st));
-  PHX_HIP(hipMemcpyAsync(W->row_ptr, A->crs_ptr.data(), sizeof(int64_t) * (W->n + 1), hipMemcpyHostToDevice, st));
-  if (nnz) {
-    PHX_HIP(hipMemcpyAsync(W->col, A->crs_col.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
-    PHX_HIP(hipMemcpyAsync(W->val, A->crs_val.data(), sizeof(T) * nnz, hipMemcpyHostToDevice, st));
+  // chunked entry layout, colour by colour (entries of a row in CRS order)
+  std::vector<int32_t> rlen(W->n);
+  std::vector<int64_t> cptr;
+  W->color_chunk.assign(W->ncolors + 1, 0);
+  int64_t slots = 0;
+  for (int c = 0; c < W->ncolors; c++) {
+    W->color_chunk[c] = (int64_t)cptr.size();
+    for (int64_t r0 = W->color_ptr[c]; r0 < W->color_ptr[c + 1]; r0 += kCarpChunk) {
+      const int64_t r1 = std::min<int64_t>(W->color_ptr[c + 1], r0 + kCarpChunk);
+      int32_t wmax = 0;
+      for (int64_t r = r0; r < r1; r++) {
+        const int64_t i = W->h_rows[r];
+        rlen[r] = (int32_t)(A->crs_ptr[i + 1] - A->crs_ptr[i]);
+        wmax = std::max(wmax, rlen[r]);
+      }
+      cptr.push_back(slots);
+      slots += (int64_t)wmax * kCarpChunk;
+    }
   }
-  sync_stream();
+  W->color_chunk[W->ncolors] = (int64_t)cptr.size();
+  cptr.push_back(slots);
+  std::vector<int32_t> ccol((size_t)std::max<int64_t>(1, slots), 0);
+  std::vector<T> cval((size_t)std::max<int64_t>(1, slots), T(0));
+  for (int c = 0; c < W->ncolors; c++)
+    for (int64_t r = W->color_ptr[c]; r < W->color_ptr[c + 1]; r++) {
+      const int64_t rl = r - W->color_ptr[c];
+      const int64_t base = cptr[W->color_chunk[c] + rl / kCarpChunk] + rl % kCarpChunk;
+      const int64_t i = W->h_rows[r];
+      for (int32_t q = 0; q < rlen[r]; q++) {
+        ccol[base + (int64_t)q * kCarpChunk] = A->crs_col[A->crs_ptr[i] + q];
+        cval[base + (int64_t)q * kCarpChunk] = A->crs_val[A->crs_ptr[i] + q];
+      }
+    }
+  hipStream_t st = rt().stream;
+  PHX_HIP(hipMalloc(&W->rows, sizeof(int32_t) * std::max<int64_t>(1, W->n)));
+  PHX_HIP(hipMalloc(&W->rlen, sizeof(int32_t) * std::max<int64_t>(1, W->n)));
+  PHX_HIP(hipMalloc(&W->chunk_ptr, sizeof(int64_t) * cptr.size()));
+  PHX_HIP(hipMalloc(&W->col, sizeof(int32_t) * ccol.size()));
+  PHX_HIP(hipMalloc(&W->val, sizeof(T) * cval.size()));
+  if (W->n) {
+    PHX_HIP(hipMemcpyAsync(W->rows, W->h_rows.data(), sizeof(int32_t) * W->n, hipMemcpyHostToDevice, st));
+    PHX_HIP(hipMemcpyAsync(W->rlen, rlen.data(), sizeof(int32_t) * W->n, hipMemcpyHostToDevice, st));
+  }
+  PHX_HIP(hipMemcpyAsync(W->chunk_ptr, cptr.data(), sizeof(int64_t) * cptr.size(), hipMemcpyHostToDevice, st));
+  PHX_HIP(hipMemcpyAsync(W->col, ccol.data(), sizeof(int32_t) * ccol.size(), hipMemcpyHostToDevice, st));
+  PHX_HIP(hipMemcpyAsync(W->val, cval.data(), sizeof(T) * cval.size(), hipMemcpyHostToDevice, st));
+  sync_stream();  // the host staging vectors die here
   return W;
 }
 
@@ -170,7 +217,8 @@ void carp_destroy(void* work) {
   if (!W) return;
   sync_stream();
   if (W->rows) PHX_HIP(hipFree(W->rows));
-  if (W->row_ptr) PHX_HIP(hipFree(W->row_ptr));
+  if (W->rlen) PHX_HIP(hipFree(W->rlen));
+  if (W->chunk_ptr) PHX_HIP(hipFree(W->chunk_ptr));
   if (W->col) PHX_HIP(hipFree(W->col));
   if (W->val) PHX_HIP(hipFree(W->val));
   delete W;
@@ -210,11 +258,12 @@ void carp_sweep(const SparseMat<T>* A, void* work, const T* sr, const T* si, con
       if (nr == 0) return;
       const int64_t total = nr * nc;
       const int grid = (int)std::min<int64_t>((total + kBlock - 1) / kBlock, (int64_t)rt().num_cu * 8);
+      const int64_t* cp = W->chunk_ptr + W->color_chunk[c];
       if (xi)
-        hipLaunchKernelGGL((k_kacz<T, true>), dim3(grid), dim3(kBlock), 0, st, W->rows + r0, nr, W->row_ptr, W->col,
-                           W->val, nc, coef, bp, ldb, xr, xi, Xr->lda);
+        hipLaunchKernelGGL((k_kacz<T, true>), dim3(grid), dim3(kBlock), 0, st, W->rows + r0, W->rlen + r0, cp, nr,
+                           W->col, W->val, nc, coef, bp, ldb, xr, xi, Xr->lda);
       else
-        hipLaunchKernelGGL((k_kacz<T, false>), dim3(grid), dim3(kBlock), 0, st, W->rows + r0, nr, W->row_ptr,
+        hipLaunchKernelGGL((k_kacz<T, false>), dim3(grid), dim3(kBlock), 0, st, W->rows + r0, W->rlen + r0, cp, nr,
                            W->col, W->val, nc, coef, bp, ldb, xr, (T*)nullptr, Xr->lda);
       PHX_HIP(hipGetLastError());
     };
