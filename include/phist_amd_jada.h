@@ -182,6 +182,16 @@ void phist_amd_DblockedMINRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, co
                                       phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[], int maxIter,
                                       int fp32, int nIter[], double relres[], int* iflag);
 
+/* Restarted GMRES(restart) on nsys <= 64 shifted systems (A + shifts[j] I) x_j = b_j of a general
+   (nonsymmetric) matrix of this library, one Krylov basis per system as in
+   src/krylov/phist_blockedgmres_def.hpp:453-1010 (restartedBlockedGMRES :1029-1200); X holds the initial
+   guess.  Converged when ||r|| / ||r0|| < tol[j] or ||r|| < 1000 eps; maxIter bounds the Arnoldi steps
+   per system.  nIter[j] = steps taken, relres[j] = final true residual over the initial one; fp32 != 0
+   iterates in FP32 on the matrix's cached FP32 copy.  iflag = number of unconverged systems. */
+void phist_amd_DblockedGMRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, const double shifts[],
+                                     phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[], int maxIter,
+                                     int restart, int fp32, int nIter[], double relres[], int* iflag);
+
 #ifdef __cplusplus
 }
 #endif

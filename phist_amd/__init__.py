@@ -296,6 +296,23 @@ class Backend:
             raise PhistError("blockedMINRES_shifted", fl.value)
         return it, rr, fl.value
 
+    def gmres_shifted(self, A, shifts, B, X, tol, max_iter, restart=30, fp32=False):
+        """phist_amd_DblockedGMRES_shifted: (A + shifts[j] I) x_j = b_j, X the initial guess;
+        returns (iters, relres, unconverged)"""
+        k = len(shifts)
+        sh = np.ascontiguousarray(shifts, np.float64)
+        tl = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, np.float64), (k,)))
+        it = np.zeros(k, np.int32)
+        rr = np.zeros(k)
+        fl = ctypes.c_int(0)
+        self.L.phist_amd_DblockedGMRES_shifted(A.h, ctypes.c_int(k), _ptr(sh), B.h, X.h, _ptr(tl),
+                                               ctypes.c_int(max_iter), ctypes.c_int(restart),
+                                               ctypes.c_int(1 if fp32 else 0), _ptr(it), _ptr(rr),
+                                               ctypes.byref(fl))
+        if fl.value < 0:
+            raise PhistError("blockedGMRES_shifted", fl.value)
+        return it, rr, fl.value
+
     def server(self, budget_bytes=200 * 2**30, slots=16):
         return Server(self, budget_bytes, slots)
 

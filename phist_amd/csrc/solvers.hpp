@@ -577,6 +577,17 @@ MinresOut<TI> minres_solve(const Operator<TI>& op, const std::vector<T>& s, cons
   return solver.solve(s, Q, B, X, tol, maxIter);
 }
 
+// restarted GMRES(m) on k shifted systems (gmres.cpp), x from X as the
+// initial guess; iters = Arnoldi steps per system, relres = ||r|| / ||r0||
+struct GmresOut {
+  std::vector<int> iters;
+  std::vector<double> relres;
+  int unconverged = 0;
+};
+template <typename T, typename TI>
+GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const Mvec<T>* B, Mvec<T>* X,
+                     const std::vector<double>& tol, int maxIter, int m);
+
 // resumable block Jacobi-Davidson on a sparse matrix of this library
 // (defined in jada.cpp): start builds the start-up basis, step runs one outer
 // iteration (true when finished), finish writes the outputs
