@@ -135,6 +135,26 @@ def test_jada_spin_chain_L14(be64, fp32):
         o.delete()
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_jada_integration_config_spinSZ14(be64, fp32):
+    """drivers/integration_tests.cmake:9-10 exactly: phist_Dsubspacejada spinSZ14 I 1 10 SR 1e-8 28 4 16 32
+    4 8 -- 10 smallest, symmetric, tol 1e-8, at most 28 outer iterations, blockSize 4, minBas 16,
+    maxBas 32, inner MINRES with at most 8 iterations (innerSolvMaxBas 8)"""
+    L = 14
+    A = be64.spinchain_sz(L, L // 2)
+    dense = ol.rows_to_dense(ol.spin_sz_rows(L, L // 2, 0), A.nrows)
+    w = np.linalg.eigvalsh(dense)
+    k = 10
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, "SR", 1e-8, block_size=4, min_bas=16, max_bas=32,
+                                                    max_iters=28, inner_max_iters=8, inner_fp32=fp32)
+    print(f"spinSZ14 integration config: {nconv} converged in {niter} iterations")
+    assert nconv >= k and niter <= 28, (nconv, niter, res)
+    check_found(ev[:k], w, "SR")
+    check_schur(be64, A, Q, R, k, 1e-8)
+    for o in (Q, R, A):
+        o.delete()
+
+
 def test_jada_rejects_unsupported(be64):
     import ctypes
     import phist_amd as pa
