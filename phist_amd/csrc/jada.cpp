@@ -221,7 +221,10 @@ class JadaRun {
       MvBox<T> wcol(map, 1);
       for (int j = 0; j < nV; j++) {
         View<T> vj(V_.get(), j, j), avj(AV_.get(), j, j);
-        A.apply_shifted(T(1), nullptr, vj.get(), T(0), avj.get(), nullptr);
+        // the SpMV gathers rows of its input: a contiguous copy instead of a
+        // column of the wide basis (one cache line per gathered entry)
+        blas_copy<T>(vj.get(), wcol.get());
+        A.apply_shifted(T(1), nullptr, wcol.get(), T(0), avj.get(), nullptr);
         if (j + 1 >= maxBase) break;
         View<T> Vj(V_.get(), 0, j), vn(V_.get(), j + 1, j + 1);
         blas_copy<T>(avj.get(), vn.get());

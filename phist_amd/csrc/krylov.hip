@@ -96,13 +96,19 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
     T vv[VPL], ww[VPL];
     vload<T, VPL>(p.v + o, vv);
     vload<T, VPL>(p.w + o, ww);
-    if (nq > 0) {  // w(i,:) -= Q(i,:) * C
+    if (nq > 0) {  // w(i,:) -= Q(i,:) * C, the Q row read 8 entries at a time (loads in flight together)
       const T* qi = p.q + i * p.ldq;
-      const T* cj = sc + c0;
-      for (int j = 0; j < nq; j++, cj += VPL * LPR) {
-        const T qv = qi[j];
+      for (int j0 = 0; j0 < nq; j0 += 8) {
+        T qv[8];
 #pragma unroll
-        for (int e = 0; e < VPL; e++) ww[e] = fma(-qv, cj[e], ww[e]);
+        for (int u = 0; u < 8; u++) qv[u] = j0 + u < nq ? qi[j0 + u] : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (j0 + u >= nq) break;
+          const T* cj = sc + (j0 + u) * (VPL * LPR) + c0;
+#pragma unroll
+          for (int e = 0; e < VPL; e++) ww[e] = fma(-qv[u], cj[e], ww[e]);
+        }
       }
     }
 #pragma unroll

@@ -489,10 +489,16 @@ __global__ void __launch_bounds__(kBlock) k_rowmix(const MixArgs<T> p) {
 #pragma unroll
             for (int e = 0; e < VPL; e++) acc[e] = fma(wr[u], sD[(j + u) * p.m + c0 + e], acc[e]);
         }
-      for (; j < p.m; j++) {
-        const T wr = wp[j];
+      for (; j < p.m; j += 8) {
+        T wr[8];
 #pragma unroll
-        for (int e = 0; e < VPL; e++) acc[e] = fma(wr, sD[j * p.m + c0 + e], acc[e]);
+        for (int u = 0; u < 8; u++) wr[u] = j + u < p.m ? wp[j + u] : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (j + u >= p.m) break;
+#pragma unroll
+          for (int e = 0; e < VPL; e++) acc[e] = fma(wr[u], sD[(j + u) * p.m + c0 + e], acc[e]);
+        }
       }
     }
     const T* vp = p.v + i * p.ldv;
@@ -506,10 +512,16 @@ __global__ void __launch_bounds__(kBlock) k_rowmix(const MixArgs<T> p) {
 #pragma unroll
           for (int e = 0; e < VPL; e++) acc[e] = fma(sC[(j + u) * p.m + c0 + e], vr[u], acc[e]);
       }
-    for (; j < p.k; j++) {
-      const T vr = vp[j];
+    for (; j < p.k; j += 8) {  // scalar loads, 8 in flight before their products
+      T vr[8];
 #pragma unroll
-      for (int e = 0; e < VPL; e++) acc[e] = fma(sC[j * p.m + c0 + e], vr, acc[e]);
+      for (int u = 0; u < 8; u++) vr[u] = j + u < p.k ? vp[j + u] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (j + u >= p.k) break;
+#pragma unroll
+        for (int e = 0; e < VPL; e++) acc[e] = fma(sC[(j + u) * p.m + c0 + e], vr[u], acc[e]);
+      }
     }
     vstore<T, VPL>(wp + c0, acc);
   }
