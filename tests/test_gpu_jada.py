@@ -52,6 +52,30 @@ def test_minres_shifted_laplace(be64, fp32):
         o.delete()
 
 
+def test_minres_shifted_large_block(be64):
+    """12 systems on a 66^3 Laplacian (287496 rows), all shifts positive:
+    the device-resident iteration on a block wider than 8 columns"""
+    n = 66
+    A = be64.laplace3d(n, n, n)
+    N = n ** 3
+    rng = np.random.default_rng(9)
+    shifts = np.linspace(0.2, 3.0, 12)
+    k = len(shifts)
+    Bh = rng.uniform(-1, 1, (N, k))
+    m = be64.map(N)
+    B, X = be64.mvec(m, k, Bh), be64.mvec(m, k)
+    tol = 1e-9
+    it, rr, unconv = be64.minres_shifted(A, shifts, B, X, tol, 3000)
+    assert unconv == 0, (it, rr)
+    Xh = X.get()
+    AX = ol.spmv(crs_of(A), 1.0, Xh, 0.0, np.zeros((N, k)))
+    true = np.linalg.norm(Bh - AX - shifts * Xh, axis=0) / np.linalg.norm(Bh, axis=0)
+    assert np.all(true <= 1e-8), true
+    assert np.all(rr <= tol)
+    for o in (B, X, A):
+        o.delete()
+
+
 def check_found(got, w, which, tol=1e-7):
     """every converged Ritz value is an eigenvalue, and the distinct values
     found are the first distinct eigenvalues in `which` order (a search space

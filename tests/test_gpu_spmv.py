@@ -286,3 +286,35 @@ def test_banded_generator_rows_and_product(be64, n, nnz, hb):
     assert np.allclose(ww, ol.dot(Yref, Yref), rtol=1e-12)
     for o in (x, y, A):
         o.delete()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [12, 16, 24])
+def test_spmv_column_blocks_large(be64, be32, dtype, nvec):
+    """300000 rows of 9 nonzeros and 12-24 columns (a persistent grid of
+    many tiles per workgroup): shifts, beta and both dot products"""
+    be = be64 if dtype == np.float64 else be32
+    n = 300000
+    A = be.banded_random(n, 9, 100, 40.0, 77)
+    ro, ci, va = A.crs()
+    ref = ol.Crs(n, n, ro, ci, va.astype(float))
+    rng = np.random.default_rng(nvec)
+    X = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    Y0 = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    shifts = rng.uniform(-2, 2, nvec).astype(dtype)
+    m = be.map(n)
+    x, y = be.mvec(m, nvec, X), be.mvec(m, nvec, Y0)
+    be.spmv_shifted(0.7, A, shifts, x, 0.3, y)
+    Xf = X.astype(float)
+    want = ol.spmv(ref, 0.7, Xf, 0.3, Y0.astype(float), shifts=shifts.astype(float))
+    tol = RTOL[dtype]
+    assert colwise_rel(y.get().astype(float), want) <= tol
+    w = be.mvec(m, nvec, Y0)
+    ww, vw = be.fused_spmv_mvdot(1.1, A, x, 0.5, w)
+    Wref = ol.spmv(ref, 1.1, Xf, 0.5, Y0.astype(float))
+    assert colwise_rel(w.get().astype(float), Wref) <= tol
+    assert np.allclose(ww, ol.dot(Wref, Wref), rtol=10 * tol, atol=0)
+    scale = ol.dot(np.abs(Xf), np.abs(Wref))
+    assert np.all(np.abs(vw - ol.dot(Xf, Wref)) <= 10 * tol * scale)
+    for o in (x, y, w, A):
+        o.delete()
