@@ -28,6 +28,17 @@ void phist_amd_set_grid_blocks(int nblocks, int* iflag);
  * blocks of >= 4 columns, whose gathers then keep a wide band's x rows
  * L2-resident, 3 for such rows and narrower blocks, 6 for shorter rows) */
 void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
+/* the device-resident MINRES iteration (blocked MINRES, the eigensolver's
+ * correction equations, the service loop): iters_per_call (>= 1, default 1)
+ * iterations are queued per host step, never more than an unfinished system
+ * may still take; enable != 0 (default 0) replays runs of graph_len (>= 1,
+ * default 4) iterations from captured HIP graphs (one per layout of the
+ * rotating blocks, captured on first use and again when the projection
+ * basis changes) instead of launching their kernels one by one.  Results are
+ * the same bit for bit either way.  stats: graphs captured and graph
+ * launches since initialisation. */
+void phist_amd_set_minres_graphs(int enable, int iters_per_call, int graph_len, int* iflag);
+void phist_amd_minres_graph_stats(int64_t* captures, int64_t* launches, int* iflag);
 
 /* the library's HIP stream (hipStream_t), device synchronisation, and an
  * event timer on that stream (ms between start and stop) */

@@ -296,6 +296,18 @@ class Backend:
             raise PhistError("blockedMINRES_shifted", fl.value)
         return it, rr, fl.value
 
+    def set_minres_graphs(self, enable, iters_per_call=1, graph_len=4):
+        """device MINRES: queue up to iters_per_call iterations per step; with enable, replay runs of
+        graph_len iterations from captured HIP graphs"""
+        self.gcall("amd_set_minres_graphs", ctypes.c_int(1 if enable else 0), ctypes.c_int(iters_per_call),
+                   ctypes.c_int(graph_len))
+
+    def minres_graph_stats(self):
+        """(graphs captured, graph launches) since initialisation"""
+        c, n = ctypes.c_int64(0), ctypes.c_int64(0)
+        self.gcall("amd_minres_graph_stats", ctypes.byref(c), ctypes.byref(n))
+        return c.value, n.value
+
     def gmres_shifted(self, A, shifts, B, X, tol, max_iter, restart=30, fp32=False):
         """phist_amd_DblockedGMRES_shifted: (A + shifts[j] I) x_j = b_j, X the initial guess;
         returns (iters, relres, unconverged)"""

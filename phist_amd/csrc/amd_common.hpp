@@ -95,8 +95,23 @@ struct Runtime {
   int gram_blocks_per_cu = 8;
   // row-mixing products (V*C, in-place V*M) with k, m >= 8 run on the matrix cores
   bool mix_mfma = true;
-  bool minres_device = true;
-  bool gram_rows = true;  // narrow unaligned Grams: several columns per lane (k_gram_rows)  // MINRES scalars on the device for this library's matrices
+  bool minres_device = true;  // MINRES scalars on the device for this library's matrices
+  // device MINRES: iterations enqueued per iterate() call (at most; fewer when
+  // no active system can take that many more), and whether runs of
+  // minres_graph_len iterations are replayed from captured HIP graphs instead
+  // of launching their kernels one by one (measured no faster on MI355X for
+  // 3k..262k rows, so off by default)
+  int minres_iters_per_call = 1;
+  bool minres_graphs = false;
+  int minres_graph_len = 4;
+  int64_t graph_captures = 0, graph_launches = 0;
+  // device_work while a graph is captured: every request is answered with
+  // work_arena (the capture records, nothing runs), the largest request is
+  // kept in work_need so that the capturer can check its arena afterwards
+  bool capturing = false;
+  void* work_arena = nullptr;
+  size_t work_need = 0;
+  bool gram_rows = true;  // narrow unaligned Grams: several columns per lane (k_gram_rows)
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;
@@ -254,6 +269,9 @@ struct SpmvRequest {
   // when set, the dots stay on the device: dots_dev[0..nvec) = ww, dots_dev[nvec..2nvec) = vw
   // (both written whichever were requested; at most 64 column groups)
   void* dots_dev = nullptr;
+  // when set (with has_shift), the shifts are read on the device from
+  // shifts_dev[0..nvec) instead of the host array
+  const void* shifts_dev = nullptr;
 };
 
 // per-column state of the device-resident MINRES recurrences (Paige & Saunders)

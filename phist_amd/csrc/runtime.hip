@@ -54,12 +54,19 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA")) r.mix_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_DEVICE")) r.minres_device = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_ROWS")) r.gram_rows = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH")) r.minres_graphs = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_MINRES_ITERS_PER_CALL")) r.minres_iters_per_call = std::max(1, atoi(s));
+  if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH_LEN")) r.minres_graph_len = std::max(1, atoi(s));
   kiss_reset();
   r.initialized = true;
 }
 
 void* device_work(size_t bytes) {
   Runtime& r = rt();
+  if (r.capturing) {
+    r.work_need = std::max(r.work_need, bytes);
+    return r.work_arena;
+  }
   if (bytes > r.d_work_bytes) {
     if (r.d_work) {
       PHX_HIP(hipStreamSynchronize(r.stream));
@@ -75,6 +82,7 @@ void* device_work(size_t bytes) {
 
 void* host_work(size_t bytes) {
   Runtime& r = rt();
+  PHX_REQUIRE(!r.capturing, PHIST_FUNCTIONAL_ERROR, "a host-side reduction cannot be part of a captured graph");
   if (bytes > r.h_work_bytes) {
     if (r.h_work) {
       PHX_HIP(hipStreamSynchronize(r.stream));
@@ -482,6 +490,25 @@ void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag) {
   guard(iflag, __func__, [&] {
     ensure_init();
     rt().spmv_blocks_per_cu = n < 0 ? -1 : n;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_minres_graphs(int enable, int iters_per_call, int graph_len, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_REQUIRE(iters_per_call >= 1 && graph_len >= 1, PHIST_INVALID_INPUT, "%d iterations per call, %d per graph",
+                iters_per_call, graph_len);
+    rt().minres_graphs = enable != 0;
+    rt().minres_iters_per_call = iters_per_call;
+    rt().minres_graph_len = graph_len;
+    *iflag = 0;
+  });
+}
+void phist_amd_minres_graph_stats(int64_t* captures, int64_t* launches, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    if (captures) *captures = rt().graph_captures;
+    if (launches) *launches = rt().graph_launches;
     *iflag = 0;
   });
 }

@@ -169,15 +169,17 @@ struct Operator {
     PHX_REQUIRE(iflag == 0, iflag, "user operator returned %d", iflag);
     if (xy_dots) blas_dot<T>(X, Y, xy_dots);
   }
-  // Y = (A + diag(s)) X with the column dots <X, Y> left on the device
-  // (dots_dev[nvec + c]; dots_dev[c] = <Y, Y>); sparse matrices of this library only
-  void apply_shifted_dev(const T* s, const Mvec<T>* X, Mvec<T>* Y, T* dots_dev) const {
+  // Y = (A + diag(s)) X with the shifts s on the device and the column dots
+  // <X, Y> left on the device (dots_dev[nvec + c]; dots_dev[c] = <Y, Y>);
+  // sparse matrices of this library only
+  void apply_shifted_dev(const T* s_dev, const Mvec<T>* X, Mvec<T>* Y, T* dots_dev) const {
     PHX_REQUIRE(A != nullptr, PHIST_NOT_IMPLEMENTED, "device dots need a sparse matrix of this library");
     SpmvRequest req;
     req.has_shift = true;
+    req.shifts_dev = s_dev;
     req.dot_vw = true;
     req.dots_dev = dots_dev;
-    spmv_launch<T>(A, T(1), s, X, T(0), Y, req, nullptr, nullptr);
+    spmv_launch<T>(A, T(1), nullptr, X, T(0), Y, req, nullptr, nullptr);
   }
 };
 
@@ -224,7 +226,8 @@ template <typename T>
 class MinresBatch {
  public:
   MinresBatch(const Operator<T>& op, const Map* map, int cap, const Mvec<T>* Q = nullptr)
-      : op_(op), map_(map), cap_(cap), Q_(Q), st_(cap), load_seq_(cap, 0) {
+      : op_(op), map_(map), cap_(cap), Q_((Q && Q->nvec > 0) ? Q : nullptr), st_(cap), load_seq_(cap, 0),
+        view_seq_(cap, 0) {
     PHX_REQUIRE(cap >= 1 && cap <= 64, PHIST_INVALID_INPUT, "MINRES batch of %d systems", cap);
     // six separate blocks of `cap` columns (row stride cap each): the SpMV
     // gathers rows of V, and a V whose rows are interleaved with the other
@@ -239,7 +242,9 @@ class MinresBatch {
       d_st_ = static_cast<MinresState*>(dev_alloc(sizeof(MinresState) * cap));
       d_dots_ = static_cast<T*>(dev_alloc(sizeof(T) * 2 * cap));
       d_work_ = static_cast<T*>(dev_alloc(sizeof(T) * 8 * cap));
+      d_shift_ = static_cast<T*>(dev_alloc(sizeof(T) * cap));
       h_buf_ = static_cast<MinresState*>(pinned_alloc(sizeof(MinresState) * 3 * cap));
+      h_shift_ = static_cast<T*>(pinned_alloc(sizeof(T) * cap));
       for (int e = 0; e < 3; e++) PHX_HIP(hipEventCreateWithFlags(&ev_[e], hipEventDisableTiming));
       upload(0, cap);  // all frozen
     }
@@ -250,6 +255,10 @@ class MinresBatch {
       for (int e = 0; e < 3; e++)
         if (ev_[e]) PHX_HIP(hipEventDestroy(ev_[e]));
       pinned_release(h_buf_, sizeof(MinresState) * 3 * cap_);  // synchronises the stream first
+      pinned_release(h_shift_, sizeof(T) * cap_);
+      drop_graphs();
+      if (garena_) dev_release(garena_, garena_bytes_);
+      dev_release(d_shift_, sizeof(T) * cap_);
       dev_release(d_work_, sizeof(T) * 8 * cap_);
       dev_release(d_dots_, sizeof(T) * 2 * cap_);
       dev_release(d_st_, sizeof(MinresState) * cap_);
@@ -261,11 +270,11 @@ class MinresBatch {
 
   int capacity() const { return cap_; }
   // projector P = I - QQ' for the following systems (Q = null: none); Q must
-  // stay alive while the batch iterates
+  // stay alive while the batch iterates.  The coefficient block Q'w is kept
+  // while Q keeps its width, so the captured iterations stay valid.
   void set_projection(const Mvec<T>* Q) {
     Q_ = (Q && Q->nvec > 0) ? Q : nullptr;
     if (Q_ && (!C_ || C_->get()->nrows != Q_->nvec)) C_.reset(new SdBox<T>(Q_->nvec, cap_));
-    if (!Q_) C_.reset();
   }
   int free_slot() const {
     for (int j = 0; j < cap_; j++)
@@ -344,7 +353,9 @@ class MinresBatch {
     if (dev_) upload(j, 1);  // frozen on the device as well
   }
 
-  // one MINRES iteration of every active slot
+  // one or more MINRES iterations of every active slot (device scalars: up to
+  // the runtime's iterations per call, never more than any active system may
+  // still take; when the queued ones suffice, only the host view catches up)
   void iterate() {
     if (dev_)
       iterate_dev();
@@ -396,9 +407,12 @@ class MinresBatch {
     if (uploads_) PHX_HIP(hipEventSynchronize(ev_[2]));  // the staging area is free again
     for (int j = j0; j < j0 + n; j++) {
       h_up()[j] = st_[j].m;
+      h_shift_[j] = st_[j].shift;
       load_seq_[j] = seq_ + 1;
+      view_seq_[j] = seq_;
     }
     PHX_HIP(hipMemcpyAsync(d_st_ + j0, h_up() + j0, sizeof(MinresState) * n, hipMemcpyHostToDevice, rt().stream));
+    PHX_HIP(hipMemcpyAsync(d_shift_ + j0, h_shift_ + j0, sizeof(T) * n, hipMemcpyHostToDevice, rt().stream));
     PHX_HIP(hipEventRecord(ev_[2], rt().stream));
     uploads_++;
   }
@@ -406,18 +420,21 @@ class MinresBatch {
     PHX_HIP(hipEventSynchronize(ev_[b]));
     const MinresState* src = h_buf_ + (size_t)b * cap_;
     for (int j = 0; j < cap_; j++)
-      if (copy_seq_[b] >= load_seq_[j]) st_[j].m = src[j];
+      if (copy_seq_[b] >= load_seq_[j]) {
+        st_[j].m = src[j];
+        view_seq_[j] = copy_seq_[b];
+      }
   }
 
-  void iterate_dev() {
+  // the kernels of one iteration on the library stream (recorded instead of
+  // run while a graph is captured); the roles of the rotating blocks advance
+  void enqueue_iteration() {
     const int k = cap_;
-    std::vector<T> s(k);
-    for (int j = 0; j < k; j++) s[j] = st_[j].shift;
     Mvec<T>* v = cur(V);
     Mvec<T>* w = cur(W);
-    op_.apply_shifted_dev(s.data(), v, w, d_dots_);
-    const bool fuse = C_ && Q_->nvec <= kFusedProjCols;  // w -= Q*C inside the Lanczos update
-    if (C_) {
+    op_.apply_shifted_dev(d_shift_, v, w, d_dots_);
+    const bool fuse = Q_ && Q_->nvec <= kFusedProjCols;  // w -= Q*C inside the Lanczos update
+    if (Q_) {
       gemm_gram<T>(T(1), Q_, w, T(0), C_->get(), false);
       if (!fuse) gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
     }
@@ -427,9 +444,116 @@ class MinresBatch {
                               fuse ? C_->get()->root->ld : 0);
     std::swap(role_[V], role_[VOLD]);
     std::swap(role_[D1], role_[D2]);
-    ++seq_;
-    const int b = (int)(seq_ & 1);
-    hipStream_t st = rt().stream;
+  }
+
+  void drop_graphs() {
+    for (hipGraphExec_t& g : gexec_)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
+
+  // glen_ iterations captured as one HIP graph, for the role layout p the
+  // blocks are in now (the layouts alternate, so two graphs cover every
+  // start); kernel arguments name only blocks, device scalars, the
+  // projection basis and its coefficient block, and work space the batch owns
+  void capture(int p) {
+    Runtime& r = rt();
+    hipStream_t st = r.stream;
+    int saved[6];
+    std::copy(role_, role_ + 6, saved);
+    for (int attempt = 0; attempt < 2; attempt++) {
+      r.work_arena = garena_;
+      r.work_need = 0;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) break;
+      r.capturing = true;
+      bool ok = true;
+      try {
+        for (int i = 0; i < glen_; i++) enqueue_iteration();
+      } catch (...) {
+        ok = false;
+      }
+      r.capturing = false;
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &g);
+      std::copy(saved, saved + 6, role_);
+      if (!ok || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        break;
+      }
+      if (r.work_need > garena_bytes_) {  // recorded with too small a work space: never launched
+        (void)hipGraphDestroy(g);
+        drop_graphs();  // the other layout's graph names the old work space
+        if (garena_) dev_release(garena_, garena_bytes_);
+        garena_bytes_ = r.work_need;
+        garena_ = dev_alloc(garena_bytes_);
+        continue;
+      }
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess || !ex) break;
+      gexec_[p] = ex;
+      r.graph_captures++;
+      return;
+    }
+    (void)hipGetLastError();
+    graph_off_ = true;  // this batch launches its kernels one by one from now on
+  }
+
+  void iterate_dev() {
+    Runtime& r = rt();
+    const int k = cap_;
+    int64_t rem = 0;  // iterations the busiest active system may still take beyond those queued
+    for (int j = 0; j < k; j++)
+      if (st_[j].m.active)
+        rem = std::max<int64_t>(rem, (int64_t)st_[j].m.max_iter - st_[j].m.iters - (seq_ - view_seq_[j]));
+    if (rem <= 0 && pending_ >= 0) {
+      drain();
+      return;
+    }
+    const int n = (int)std::max<int64_t>(1, std::min<int64_t>(r.minres_iters_per_call, rem));
+    const bool graphs = r.minres_graphs && !graph_off_ && !r.ktimer_on && cur(V)->nrows > 0 &&
+                        (!Q_ || Q_->nvec <= kFusedProjCols);
+    if (graphs) {  // the graphs hold the projection's pointers and width
+      const T* qd = Q_ ? Q_->dev() : nullptr;
+      const int nq = Q_ ? Q_->nvec : 0;
+      const int64_t ldq = Q_ ? Q_->lda : 0;
+      const T* cd = Q_ ? C_->get()->dev() : nullptr;
+      const int glen = std::max(1, r.minres_graph_len);
+      if (qd != gq_ || nq != gnq_ || ldq != gldq_ || cd != gc_ || glen != glen_) {
+        drop_graphs();
+        gq_ = qd;
+        gnq_ = nq;
+        gldq_ = ldq;
+        gc_ = cd;
+        glen_ = glen;
+      }
+    }
+    for (int i = 0; i < n;) {
+      if (graphs && !graph_off_ && n - i >= glen_) {
+        const int p = role_[V];  // 0 or 1
+        if (!gexec_[p]) capture(p);
+        if (gexec_[p]) {
+          PHX_HIP(hipGraphLaunch(gexec_[p], r.stream));
+          r.graph_launches++;
+          for (int t = 0; t < glen_; t++) {
+            std::swap(role_[V], role_[VOLD]);
+            std::swap(role_[D1], role_[D2]);
+          }
+          i += glen_;
+          continue;
+        }
+      }
+      enqueue_iteration();
+      i++;
+    }
+    // a copy of the state after these iterations; the host view takes the
+    // previous call's copy (lagging one call behind the queue)
+    seq_ += n;
+    const int b = pending_ == 0 ? 1 : 0;
+    hipStream_t st = r.stream;
     PHX_HIP(hipMemcpyAsync(h_buf_ + (size_t)b * cap_, d_st_, sizeof(MinresState) * cap_, hipMemcpyDeviceToHost, st));
     PHX_HIP(hipEventRecord(ev_[b], st));
     copy_seq_[b] = seq_;
@@ -444,7 +568,7 @@ class MinresBatch {
     Mvec<T>* v = cur(V);
     Mvec<T>* w = cur(W);
     op_.apply_shifted(T(1), s.data(), v, T(0), w, alpha.data());
-    if (C_) {
+    if (Q_) {
       gemm_gram<T>(T(1), Q_, w, T(0), C_->get());
       gemm_times_sd<T>(T(-1), Q_, C_->get(), T(1), w);
     }
@@ -505,11 +629,24 @@ class MinresBatch {
   MinresState* d_st_ = nullptr;
   T* d_dots_ = nullptr;
   T* d_work_ = nullptr;
+  T* d_shift_ = nullptr;
   MinresState* h_buf_ = nullptr;  // pinned: two lagged state copies, then the upload staging
+  T* h_shift_ = nullptr;          // pinned staging of the shifts
   hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};  // two lagged state copies, the last upload
-  int64_t seq_ = 0, copy_seq_[2] = {0, 0}, uploads_ = 0;
+  int64_t seq_ = 0, copy_seq_[2] = {0, 0}, uploads_ = 0;  // seq_: iterations queued so far
   int pending_ = -1;
-  std::vector<int64_t> load_seq_;
+  std::vector<int64_t> load_seq_;  // slot j's state was uploaded before iteration load_seq_[j]
+  std::vector<int64_t> view_seq_;  // the host view of slot j is its state after view_seq_[j] iterations
+  // captured iterations, one per role layout, and what they were captured with
+  hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+  const T* gq_ = nullptr;
+  int gnq_ = 0;
+  int64_t gldq_ = 0;
+  const T* gc_ = nullptr;
+  int glen_ = 1;  // iterations per graph
+  void* garena_ = nullptr;  // device work space of the captured kernels
+  size_t garena_bytes_ = 0;
+  bool graph_off_ = false;
 };
 
 template <typename T>

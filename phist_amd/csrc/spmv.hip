@@ -44,6 +44,7 @@ struct SpmvArgs {
   int flags;
   int blocks_per_cu;  // host side: persistent grid size per CU (0 = as many as fit)
   T* __restrict__ partials;  // [gridDim.x][2][ncols]
+  const T* __restrict__ shift_dev;  // shifts on the device (this launch's columns), else shift[]
   T shift[64 * 4];
 };
 
@@ -115,7 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 
   T sh[VPL];
 #pragma unroll
-  for (int e = 0; e < VPL; e++) sh[e] = (colok && (flags & F_SHIFT)) ? a.shift[c0 + e] : T(0);
+  for (int e = 0; e < VPL; e++)
+    sh[e] = (colok && (flags & F_SHIFT)) ? (a.shift_dev ? a.shift_dev[c0 + e] : a.shift[c0 + e]) : T(0);
   T dww[VPL], dvw[VPL];
 #pragma unroll
   for (int e = 0; e < VPL; e++) dww[e] = dvw[e] = T(0);
@@ -326,7 +328,8 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
     a.x = x->dev() + cb;
     a.y = y->dev() + cb;
     a.u = U ? U->dev() + cb : nullptr;
-    for (int c = 0; c < nc; c++) a.shift[c] = req.has_shift ? shifts_host[cb + c] : T(0);
+    a.shift_dev = (req.has_shift && req.shifts_dev) ? static_cast<const T*>(req.shifts_dev) + cb : nullptr;
+    for (int c = 0; c < nc; c++) a.shift[c] = (req.has_shift && !a.shift_dev) ? shifts_host[cb + c] : T(0);
     const int groups = (nc + VPL - 1) / VPL;
     int lpr = 1;
     while (lpr < groups) lpr <<= 1;
