@@ -399,15 +399,17 @@ class JadaRun {
       }
       // extend the basis: V(:, nV:nV+k) = orthonormalised t
       View<T> Vful(V_.get(), 0, nV - 1), Vv(V_.get(), nV, nV + k - 1), AVv(AV_.get(), nV, nV + k - 1);
-      blas_copy<T>(t.get(), Vv.get());
       SdBox<T> R1(k, k), R2(nV, k);
       int rank = 0;
-      // no rank truncation here: near convergence t is tiny, and a rank
-      // test would replace it by noise (the reference passes no flags)
-      api::orthog<T>(Vful.get(), Vv.get(), R1.get(), R2.get(), 3, &rank, 0);
-      // the SpMV gathers rows of its input: take them from the narrow block t
-      // (row stride bs) rather than from the wide basis (row stride maxBas)
-      blas_copy<T>(Vv.get(), t.get());
+      // orthonormalise the narrow block t (row stride bs) and copy it into
+      // the basis afterwards: the Gram and in-place updates of a k-column
+      // view of the wide basis (row stride maxBas) would move a whole cache
+      // line per row for k*8 useful bytes.  No rank truncation: near
+      // convergence t is tiny, and a rank test would replace it by noise
+      // (the reference passes no flags)
+      api::orthog<T>(Vful.get(), t.get(), R1.get(), R2.get(), 3, &rank, 0);
+      blas_copy<T>(t.get(), Vv.get());
+      // the SpMV gathers rows of its input: take them from t as well
       A.apply_shifted(T(1), nullptr, t.get(), T(0), AVv.get(), nullptr);
       View<T> Vall(V_.get(), 0, nV + k - 1);
       SdBox<T> Hn(nV + k, k);
