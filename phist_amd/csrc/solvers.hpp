@@ -668,18 +668,21 @@ class MinresSolver {
                       const std::vector<double>& tol, int maxIter) {
     const int k = B->nvec;
     const Mvec<TI>* Qi = nullptr;
-    std::unique_ptr<View<TI>> qv;
     if (Q && Q->nvec > 0) {
-      if constexpr (std::is_same<T, TI>::value) {
-        Qi = Q;
-      } else {
-        if (!q_.p || q_->nvec < Q->nvec) {
+      // every iteration reads Q twice (Gram and projection); a view that
+      // skips most of each row of its parent is first copied into a block of
+      // its own width (as is a basis of another precision)
+      bool copy = 2 * Q->nvec <= Q->lda;
+      if constexpr (!std::is_same<T, TI>::value) copy = true;
+      if (copy) {
+        if (!q_.p || q_->nvec != Q->nvec) {
           MvBox<TI> fresh(map_, Q->nvec);
           std::swap(q_.p, fresh.p);
         }
-        qv.reset(new View<TI>(q_.get(), 0, Q->nvec - 1));
-        copy_any<T, TI>(Q, qv->get());
-        Qi = qv->get();
+        copy_any<T, TI>(Q, q_.get());
+        Qi = q_.get();
+      } else {
+        if constexpr (std::is_same<T, TI>::value) Qi = Q;
       }
     }
     mb_.set_projection(Qi);
