@@ -90,6 +90,11 @@ struct Runtime {
   int spmv_blocks_per_cu = -1;  // < 0: spmv_table_blocks_per_cu
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
+  // ... when the wider side has at least gram_mfma_wide columns, or an
+  // operand is a column view of a wider block, or a self Gram has >= 8
+  // (both sides zero-padded to 16 on the matrix cores; measured on 2.7M rows:
+  // the VALU kernels win only for compact blocks of <= 8 columns)
+  int gram_mfma_wide = 9;
   // workgroups per CU of the Gram kernels (each wave leaves one partial tile,
   // so this also bounds the finalize work)
   int gram_blocks_per_cu = 8;

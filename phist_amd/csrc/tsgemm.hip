@@ -385,10 +385,12 @@ void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C,
   const int ldc = C->root->ld;
   // the sliced operand is the wider one; the broadcast side is cut in blocks
   // of at most 32 columns (each block one pass over the rows)
-  if (m >= 16 && k >= 16 && rt().gram_mfma) {
+  const bool self = V->root == W->root && V->jofs == W->jofs && m == k;
+  const int wide = std::max(m, k);
+  const bool strided = V->lda != m || W->lda != k;
+  if (rt().gram_mfma && (wide >= rt().gram_mfma_wide || (strided && wide >= 4) || (self && wide >= 8))) {
     // matrix cores: 32 x 32 output blocks (the self Gram computes the upper
     // blocks and mirrors them)
-    const bool self = V->root == W->root && V->jofs == W->jofs && m == k;
     for (int i0 = 0; i0 < m; i0 += 32)
       for (int j0 = 0; j0 < k; j0 += 32) {
         const int mi = std::min(32, m - i0), kj = std::min(32, k - j0);
