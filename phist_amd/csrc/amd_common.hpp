@@ -100,6 +100,10 @@ struct Runtime {
   int gram_blocks_per_cu = 8;
   // row-mixing products (V*C, in-place V*M) with k, m >= 8 run on the matrix cores
   bool mix_mfma = true;
+  // ... for at least this many output columns (and 8..64 inputs); narrow
+  // outputs are zero-padded to 16 (measured faster than the VALU kernels on
+  // 2.7M rows for 1..4 output columns from 8..44 inputs, FP64 and FP32)
+  int mix_mfma_min_m = 1;
   bool minres_device = true;  // MINRES scalars on the device for this library's matrices
   // device MINRES: iterations enqueued per iterate() call (at most; fewer when
   // no active system can take that many more), and whether runs of

@@ -51,6 +51,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
+  if (const char* s = getenv("PHIST_AMD_MIX_MFMA_MIN_M")) r.mix_mfma_min_m = std::max(1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_GRAM_BLOCKS_PER_CU")) r.gram_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA")) r.mix_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_DEVICE")) r.minres_device = atoi(s) != 0;

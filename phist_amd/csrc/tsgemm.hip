@@ -749,7 +749,7 @@ int mix_gram_tm(MixArgs<T>& p, bool self) {
 // modes 0 and 2 with 8 <= k, m <= 64 go to the matrix cores
 template <typename T>
 bool mix_mfma_ok(int mode, int k, int m) {
-  return rt().mix_mfma && mode != 1 && k >= 8 && m >= 8 && k <= 64 && m <= 64;
+  return rt().mix_mfma && mode != 1 && k >= 8 && m >= rt().mix_mfma_min_m && k <= 64 && m <= 64;
 }
 
 template <typename T>
@@ -863,7 +863,7 @@ void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>
   for (int c0 = 0; c0 < m; c0 += mb) {
     const int mc = std::min(mb, m - c0);
     int kb = std::max(1, (int)(kMixLds / sizeof(T)) / std::max(1, mc));
-    if (rt().mix_mfma && mc >= 8 && k > 64) kb = std::min(kb, 64);  // k blocks the matrix-core kernel takes
+    if (rt().mix_mfma && mc >= rt().mix_mfma_min_m && k > 64) kb = std::min(kb, 64);  // k blocks the matrix-core kernel takes
     for (int j0 = 0; j0 < k || (k == 0 && j0 == 0); j0 += kb) {
       const int kc = std::min(kb, k - j0);
       const T b = j0 == 0 ? beta : T(1);
