@@ -218,28 +218,37 @@ class JadaRun {
         nrm = T(1) / std::sqrt(nrm);
         blas_scale_cols<T>(v0.get(), &nrm, true);
       }
-      MvBox<T> wcol(map, 1);
+      // v_j and A v_j live in contiguous single-column blocks (wcol, acol):
+      // the SpMV gathers rows of its input, and the orthogonalisation reads
+      // and rewrites the new vector several times, where a column of the wide
+      // basis would cost a whole cache line per row for 8 useful bytes; each
+      // is copied into the basis once
+      MvBox<T> wcol(map, 1), acol(map, 1), ncol(map, 1);
+      {
+        View<T> v0(V_.get(), 0, 0);
+        blas_copy<T>(v0.get(), wcol.get());
+      }
       for (int j = 0; j < nV; j++) {
-        View<T> vj(V_.get(), j, j), avj(AV_.get(), j, j);
-        // the SpMV gathers rows of its input: a contiguous copy instead of a
-        // column of the wide basis (one cache line per gathered entry)
-        blas_copy<T>(vj.get(), wcol.get());
-        A.apply_shifted(T(1), nullptr, wcol.get(), T(0), avj.get(), nullptr);
+        View<T> avj(AV_.get(), j, j);
+        A.apply_shifted(T(1), nullptr, wcol.get(), T(0), acol.get(), nullptr);
+        blas_copy<T>(acol.get(), avj.get());
         if (j + 1 >= maxBase) break;
         View<T> Vj(V_.get(), 0, j), vn(V_.get(), j + 1, j + 1);
-        blas_copy<T>(avj.get(), vn.get());
+        blas_copy<T>(acol.get(), ncol.get());
         SdBox<T> R1(1, 1), R2(j + 1, 1);
         int rank = 0;
         // an invariant subspace is continued with a random orthogonal vector
         // (phist_simple_arnoldi_def.hpp:118-141)
-        const int dim0 = api::orthog<T>(Vj.get(), vn.get(), R1.get(), R2.get(), 3, &rank, 131072);
+        const int dim0 = api::orthog<T>(Vj.get(), ncol.get(), R1.get(), R2.get(), 3, &rank, 131072);
+        blas_copy<T>(ncol.get(), vn.get());
         for (int i = 0; i <= j; i++) H(i, j) = R2->h(i, 0);
         H(j + 1, j) = R1->h(0, 0);
         if (dim0 == 0) {  // a more accurate subdiagonal entry: v_{j+1}' A v_j
           T h;
-          blas_dot<T>(vn.get(), avj.get(), &h);
+          blas_dot<T>(ncol.get(), acol.get(), &h);
           H(j + 1, j) = h;
         }
+        std::swap(wcol.p, ncol.p);  // v_{j+1} is the next input
       }
       // H = V'AV over the first nV columns (the square part of the Arnoldi
       // Hessenberg); symmetrise exactly
@@ -361,9 +370,16 @@ class JadaRun {
     {
       const int oldConv = nConv - nNewConv;
       MvBox<T> rhs(Qout->map, k);
-      for (int i = 0; i < k; i++) {
-        View<T> src(res_.get(), selected[i] - oldConv, selected[i] - oldConv), dst(rhs.get(), i, i);
-        blas_copy<T>(src.get(), dst.get());
+      bool run = true;  // selected pairs are usually consecutive: one block copy
+      for (int i = 1; i < k; i++) run = run && selected[i] == selected[0] + i;
+      if (run) {
+        View<T> src(res_.get(), selected[0] - oldConv, selected[0] - oldConv + k - 1);
+        blas_copy<T>(src.get(), rhs.get());
+      } else {
+        for (int i = 0; i < k; i++) {
+          View<T> src(res_.get(), selected[i] - oldConv, selected[i] - oldConv), dst(rhs.get(), i, i);
+          blas_copy<T>(src.get(), dst.get());
+        }
       }
       const int k0 = opts.innerSolvMaxProjectionSpace >= 0 ? std::max(0, k_ - opts.innerSolvMaxProjectionSpace) : 0;
       View<T> Qtil(Q_.get(), k0, k_);
