@@ -81,6 +81,13 @@ struct Runtime {
   // pinned host scratch for reduction results
   void* h_work = nullptr;
   size_t h_work_bytes = 0;
+  // pinned staging slots of stage_upload_2d, each with the event of its last copy
+  static constexpr int kStageSlots = 64;
+  static constexpr size_t kStageSlotBytes = 64 * 1024;
+  char* stage_base = nullptr;
+  hipEvent_t stage_ev[kStageSlots] = {};
+  bool stage_rec[kStageSlots] = {};
+  int stage_next = 0;
   // default SELL parameters for newly built matrices (0 = automatic)
   int sell_C = 0, sell_sigma = 0;
   // maximum persistent grid size for streaming kernels (0 = automatic)
@@ -133,6 +140,11 @@ void ensure_init();
 void* device_work(size_t bytes);
 void* host_work(size_t bytes);
 void sync_stream();
+// Queues a 2-D host-to-device copy (cols columns of row_bytes bytes, pitches
+// in bytes) from a pinned staging slot holding a copy of the source, so the
+// caller may change or free its array at once; false when the block does not
+// fit a slot (nothing queued)
+bool stage_upload_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t row_bytes, size_t cols);
 
 // Caching allocators for block vectors and small dense matrices.  Released
 // device blocks go back to a size-class cache and are handed out again

@@ -223,11 +223,13 @@ void blas_put_value(Mvec<T>* V, T value) {
   // X is a single row of `value`s broadcast to every row (ldx = 0)
   std::vector<T> rowv(V->nvec, value);
   T* d_row = (T*)device_work(sizeof(T) * V->nvec);
-  PHX_HIP(hipMemcpyAsync(d_row, rowv.data(), sizeof(T) * V->nvec, hipMemcpyHostToDevice, rt().stream));
+  const size_t bytes = sizeof(T) * V->nvec;
+  const bool staged = stage_upload_2d(d_row, bytes, rowv.data(), bytes, bytes, 1);
+  if (!staged) PHX_HIP(hipMemcpyAsync(d_row, rowv.data(), bytes, hipMemcpyHostToDevice, rt().stream));
   run_rows<T>(
       K_AXPBY, d_row, 0, V->dev(), V->lda, V->nrows, V->nvec, T(0), B_READX, [](int) { return T(1); }, zero_fn<T>,
       nullptr);
-  sync_stream();  // rowv (the copy source) is released on return
+  if (!staged) sync_stream();  // rowv (the copy source) is released on return
 }
 
 template <typename T>

@@ -330,8 +330,11 @@ template <typename T>
 void sd_to_device(const SdMat<T>* M) {
   if (M->nrows == 0 || M->ncols == 0) return;
   const int ld = M->root->ld;
-  PHX_HIP(hipMemcpy2DAsync(M->dev(), ld * sizeof(T), M->host(), ld * sizeof(T), M->nrows * sizeof(T), M->ncols,
-                           hipMemcpyHostToDevice, rt().stream));
+  const size_t pitch = ld * sizeof(T);
+  if (stage_upload_2d(M->dev(), pitch, M->host(), pitch, M->nrows * sizeof(T), M->ncols)) return;
+  // larger than a staging slot: copy from the host array and wait for it
+  PHX_HIP(hipMemcpy2DAsync(M->dev(), pitch, M->host(), pitch, M->nrows * sizeof(T), M->ncols, hipMemcpyHostToDevice,
+                           rt().stream));
   sync_stream();
 }
 
@@ -339,8 +342,12 @@ template <typename T>
 void sd_from_device(const SdMat<T>* M) {
   if (M->nrows == 0 || M->ncols == 0) return;
   const int ld = M->root->ld;
-  PHX_HIP(hipMemcpy2DAsync(M->host(), ld * sizeof(T), M->dev(), ld * sizeof(T), M->nrows * sizeof(T), M->ncols,
+  if (M->nrows == ld)  // whole columns: one contiguous copy
+    PHX_HIP(hipMemcpyAsync(M->host(), M->dev(), sizeof(T) * ((size_t)ld * (M->ncols - 1) + M->nrows),
                            hipMemcpyDeviceToHost, rt().stream));
+  else
+    PHX_HIP(hipMemcpy2DAsync(M->host(), ld * sizeof(T), M->dev(), ld * sizeof(T), M->nrows * sizeof(T), M->ncols,
+                             hipMemcpyDeviceToHost, rt().stream));
   sync_stream();
 }
 

@@ -100,6 +100,35 @@ void* host_work(size_t bytes) {
 
 void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
 
+bool stage_upload_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t row_bytes, size_t cols) {
+  Runtime& r = rt();
+  if (cols == 0 || row_bytes == 0) return true;
+  const size_t need = spitch * (cols - 1) + row_bytes;
+  if (need > Runtime::kStageSlotBytes || r.capturing) return false;
+  if (!r.stage_base) {
+    void* p = nullptr;
+    PHX_HIP(hipHostMalloc(&p, Runtime::kStageSlots * Runtime::kStageSlotBytes, hipHostMallocDefault));
+    r.stage_base = static_cast<char*>(p);
+    for (int s = 0; s < Runtime::kStageSlots; s++)
+      PHX_HIP(hipEventCreateWithFlags(&r.stage_ev[s], hipEventDisableTiming));
+  }
+  const int s = r.stage_next;
+  r.stage_next = (s + 1) % Runtime::kStageSlots;
+  if (r.stage_rec[s]) PHX_HIP(hipEventSynchronize(r.stage_ev[s]));  // normally long done
+  char* slot = r.stage_base + (size_t)s * Runtime::kStageSlotBytes;
+  const char* in = static_cast<const char*>(src);
+  if (row_bytes == spitch && spitch == dpitch) {
+    memcpy(slot, in, need);
+    PHX_HIP(hipMemcpyAsync(dst, slot, need, hipMemcpyHostToDevice, r.stream));
+  } else {
+    for (size_t j = 0; j < cols; j++) memcpy(slot + j * spitch, in + j * spitch, row_bytes);
+    PHX_HIP(hipMemcpy2DAsync(dst, dpitch, slot, spitch, row_bytes, cols, hipMemcpyHostToDevice, r.stream));
+  }
+  PHX_HIP(hipEventRecord(r.stage_ev[s], r.stream));
+  r.stage_rec[s] = true;
+  return true;
+}
+
 namespace {
 // size classes: powers of two up to 2 MiB, then multiples of 2 MiB
 size_t size_class(size_t bytes) {
@@ -330,6 +359,10 @@ void phist_kernels_finalize(int* iflag) {
       pool_trim();
       if (r.d_work) PHX_HIP(hipFree(r.d_work));
       if (r.h_work) PHX_HIP(hipHostFree(r.h_work));
+      if (r.stage_base) {
+        for (int s = 0; s < Runtime::kStageSlots; s++) PHX_HIP(hipEventDestroy(r.stage_ev[s]));
+        PHX_HIP(hipHostFree(r.stage_base));
+      }
       PHX_HIP(hipEventDestroy(r.ev_start));
       PHX_HIP(hipEventDestroy(r.ev_stop));
       PHX_HIP(hipEventDestroy(r.kt_start));
