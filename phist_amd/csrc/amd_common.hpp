@@ -221,6 +221,8 @@ struct SdMat {
 };
 template <typename T> void sd_to_device(const SdMat<T>* M);
 template <typename T> void sd_from_device(const SdMat<T>* M);
+// queues the device-to-host copy only: the host copy is valid after the next stream synchronisation
+template <typename T> void sd_from_device_async(const SdMat<T>* M);
 
 // ----------------------------------------------------------------------------
 // sparse matrices

@@ -340,6 +340,12 @@ void sd_to_device(const SdMat<T>* M) {
 
 template <typename T>
 void sd_from_device(const SdMat<T>* M) {
+  sd_from_device_async(M);
+  sync_stream();
+}
+
+template <typename T>
+void sd_from_device_async(const SdMat<T>* M) {
   if (M->nrows == 0 || M->ncols == 0) return;
   const int ld = M->root->ld;
   if (M->nrows == ld)  // whole columns: one contiguous copy
@@ -348,7 +354,6 @@ void sd_from_device(const SdMat<T>* M) {
   else
     PHX_HIP(hipMemcpy2DAsync(M->host(), ld * sizeof(T), M->dev(), ld * sizeof(T), M->nrows * sizeof(T), M->ncols,
                              hipMemcpyDeviceToHost, rt().stream));
-  sync_stream();
 }
 
 namespace api {
@@ -488,6 +493,7 @@ void sdMat_add(T alpha, const SdMat<T>* A, bool ta, T beta, SdMat<T>* B) {
   template T* Mvec<T>::host();                                                           \
   template void sd_to_device<T>(const SdMat<T>*);                                        \
   template void sd_from_device<T>(const SdMat<T>*);                                      \
+  template void sd_from_device_async<T>(const SdMat<T>*);                                \
   namespace api {                                                                        \
   template Mvec<T>* mvec_create<T>(const Map*, int);                                     \
   template void mvec_delete<T>(Mvec<T>*);                                                \
