@@ -295,12 +295,21 @@ struct SpmvRequest {
   // when set (with has_shift), the shifts are read on the device from
   // shifts_dev[0..nvec) instead of the host array
   const void* shifts_dev = nullptr;
+  // when set, x is read as x(i, c) * xscale[c * xscale_stride] (a per-column
+  // factor on the device, converted to the scalar type)
+  const double* xscale = nullptr;
+  int64_t xscale_stride = 1;
 };
 
 // per-column state of the device-resident MINRES recurrences (Paige & Saunders)
 struct MinresState {
   double beta1, beta, c1, s1, c2, s2, phibar, relres, tol;
   int iters, max_iter, active, pad;
+  // the Lanczos vectors are kept unnormalised (the block that received A v):
+  // v = block * sc[...]; with iteration counter p, sc[p % 3] belongs to the
+  // current v, sc[(p + 2) % 3] to the previous one, and the rotation of
+  // iteration p writes sc[(p + 1) % 3] for the next
+  double sc[3];
 };
 
 // y = alpha*(A + diag-shifts)*x + beta*y over column range of the given views;

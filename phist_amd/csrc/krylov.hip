@@ -45,6 +45,11 @@ struct KrylovArgs {
   int nq;
   const T* cmat;
   int ldcm;
+  // optional per-column factors of v and vold (v(i,c) * scv[c * sc_stride]):
+  // the device MINRES keeps its Lanczos vectors unnormalised
+  const double* scv = nullptr;
+  const double* scvold = nullptr;
+  int64_t sc_stride = 0;
   T c0[kKrylovMaxCols], c1[kKrylovMaxCols], c2[kKrylovMaxCols], c3[kKrylovMaxCols], c4[kKrylovMaxCols];
 };
 
@@ -61,7 +66,7 @@ __device__ inline T kcoef(const KrylovArgs<T>& p, int q, int c) {
   }
 }
 
-enum : int { KF_VOLD = 1, KF_D2 = 2 };
+enum : int { KF_VOLD = 1, KF_D2 = 2, KF_NOVOLD = 4 };  // KF_NOVOLD: k_minres writes no vold
 
 template <typename T, int VPL, int LPR>
 __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
@@ -81,12 +86,15 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
     }
     __syncthreads();
   }
-  T a[VPL], b[VPL], acc[VPL];
+  T a[VPL], b[VPL], acc[VPL], sv[VPL], svo[VPL];
+  const bool scaled = p.scv != nullptr;
 #pragma unroll
   for (int e = 0; e < VPL; e++) {
     a[e] = kcoef(p, 0, c0 + e);
     b[e] = kcoef(p, 1, c0 + e);
     acc[e] = T(0);
+    sv[e] = (scaled && colok) ? (T)p.scv[(int64_t)(c0 + e) * p.sc_stride] : T(1);
+    svo[e] = (scaled && colok) ? (T)p.scvold[(int64_t)(c0 + e) * p.sc_stride] : T(1);
   }
   const TileWalk tw = xcd_tiles(p.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
@@ -96,6 +104,10 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
     T vv[VPL], ww[VPL];
     vload<T, VPL>(p.v + o, vv);
     vload<T, VPL>(p.w + o, ww);
+    if (scaled) {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) vv[e] *= sv[e];
+    }
     if (nq > 0) {  // w(i,:) -= Q(i,:) * C, the Q row read 8 entries at a time (loads in flight together)
       const T* qi = p.q + i * p.ldq;
       for (int j0 = 0; j0 < nq; j0 += 8) {
@@ -116,6 +128,10 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
     if (p.flags & KF_VOLD) {
       T vo[VPL];
       vload<T, VPL>(p.vold + o, vo);
+      if (scaled) {
+#pragma unroll
+        for (int e = 0; e < VPL; e++) vo[e] *= svo[e];
+      }
 #pragma unroll
       for (int e = 0; e < VPL; e++) ww[e] = fma(-b[e], vo[e], ww[e]);
     }
@@ -145,7 +161,8 @@ __global__ void __launch_bounds__(kBlock) k_minres(const KrylovArgs<T> p) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = lane % LPR, rw = lane / LPR, c0 = q * VPL;
   if (c0 >= p.ncols) return;
-  T dl[VPL], ep[VPL], ig[VPL], ta[VPL], ib[VPL];
+  T dl[VPL], ep[VPL], ig[VPL], ta[VPL], ib[VPL], sv[VPL];
+  const bool scaled = p.scv != nullptr;
 #pragma unroll
   for (int e = 0; e < VPL; e++) {
     dl[e] = kcoef(p, 0, c0 + e);
@@ -153,6 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_minres(const KrylovArgs<T> p) {
     ig[e] = kcoef(p, 2, c0 + e);
     ta[e] = kcoef(p, 3, c0 + e);
     ib[e] = kcoef(p, 4, c0 + e);
+    sv[e] = scaled ? (T)p.scv[(int64_t)(c0 + e) * p.sc_stride] : T(1);
   }
   const TileWalk tw = xcd_tiles(p.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
@@ -162,6 +180,10 @@ __global__ void __launch_bounds__(kBlock) k_minres(const KrylovArgs<T> p) {
     T vv[VPL], d1[VPL], dn[VPL], xx[VPL], ww[VPL];
     vload<T, VPL>(p.v + o, vv);
     vload<T, VPL>(p.d1 + o, d1);
+    if (scaled) {
+#pragma unroll
+      for (int e = 0; e < VPL; e++) vv[e] *= sv[e];
+    }
 #pragma unroll
     for (int e = 0; e < VPL; e++) dn[e] = fma(-dl[e], d1[e], vv[e]);
     if (p.flags & KF_D2) {
@@ -177,10 +199,12 @@ __global__ void __launch_bounds__(kBlock) k_minres(const KrylovArgs<T> p) {
 #pragma unroll
     for (int e = 0; e < VPL; e++) xx[e] = fma(ta[e], dn[e], xx[e]);
     vstore<T, VPL>(p.x + o, xx);
-    vload<T, VPL>(p.w + o, ww);
+    if (!(p.flags & KF_NOVOLD)) {
+      vload<T, VPL>(p.w + o, ww);
 #pragma unroll
-    for (int e = 0; e < VPL; e++) ww[e] *= ib[e];
-    vstore<T, VPL>(p.vold + o, ww);
+      for (int e = 0; e < VPL; e++) ww[e] *= ib[e];
+      vstore<T, VPL>(p.vold + o, ww);
+    }
   }
 }
 
@@ -211,13 +235,16 @@ __global__ void k_minres_ab(const T* __restrict__ dots, const MinresState* __res
 // meets the two previous rotations, a new rotation annihilates beta_{k+1};
 // writes the k_minres coefficients (delta, eps, 1/gamma, tau, 1/beta_{k+1})
 // and freezes columns that converged, broke down or hit their limit.
+// With phase >= 0 the factor of the next (unnormalised) Lanczos vector,
+// 1/beta_{k+1} or 0 for a column that was frozen, goes to sc[(phase + 1) % 3].
 template <typename T>
 __global__ void k_minres_rot(const T* __restrict__ dots, const T* __restrict__ nrm2, MinresState* __restrict__ st,
-                             int k, T* __restrict__ coef) {
+                             int k, T* __restrict__ coef, int phase) {
   const int c = threadIdx.x;
   if (c >= k) return;
   MinresState q = st[c];
   T dl = T(0), ep = T(0), ig = T(0), ta = T(0), ib = T(0);
+  double scn = 0.0;
   if (q.active) {
     const T bnext = sqrt(nrm2[c]);
     const double bk = q.iters > 0 ? q.beta : 0.0, ak = (double)dots[k + c], bn = (double)bnext;
@@ -236,6 +263,7 @@ __global__ void k_minres_rot(const T* __restrict__ dots, const T* __restrict__ n
       ig = (T)(1.0 / gamma);
       ta = (T)tau;
       ib = bn > 0.0 ? (T)(1.0 / bn) : T(0);
+      scn = bn > 0.0 ? 1.0 / bn : 0.0;
       q.c2 = q.c1;
       q.s2 = q.s1;
       q.c1 = cs;
@@ -245,8 +273,9 @@ __global__ void k_minres_rot(const T* __restrict__ dots, const T* __restrict__ n
       q.relres = fabs(q.phibar) / q.beta1;
       if (q.relres <= q.tol || bn == 0.0 || q.iters >= q.max_iter) q.active = 0;
     }
-    st[c] = q;
   }
+  if (phase >= 0) q.sc[(phase + 1) % 3] = scn;
+  st[c] = q;
   coef[c] = dl;
   coef[k + c] = ep;
   coef[2 * k + c] = ig;
@@ -378,11 +407,14 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
 // and the column states st, it forms w -= a.*v + b.*vold with the new norms,
 // advances the recurrences on the device and updates d2, x and vold.
 // work: device scratch of 7*k values of T (a, b, then the five k_minres
-// coefficients) plus k squared norms.
+// coefficients) plus k squared norms.  With phase >= 0 the Lanczos vectors
+// are unnormalised: v and vold are read times their factors in the column
+// states (sc[phase % 3], sc[(phase + 2) % 3]), w itself becomes the next v
+// (factor sc[(phase + 1) % 3]) and no vold is written.
 template <typename T>
 void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
                             const T* dots, MinresState* st, T* work, const T* Q, int64_t ldq, int nq, const T* C,
-                            int ldc) {
+                            int ldc, int phase) {
   PHX_REQUIRE(k >= 1 && k <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns", k);
   if (nrows == 0) return;  // every column starts frozen (zero right-hand side)
   hipStream_t s = rt().stream;
@@ -408,15 +440,21 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   p.nq = Q ? nq : 0;
   p.cmat = C;
   p.ldcm = ldc;
+  if (phase >= 0) {
+    p.scv = &st[0].sc[phase % 3];
+    p.scvold = &st[0].sc[(phase + 2) % 3];
+    p.sc_stride = sizeof(MinresState) / sizeof(double);
+  }
   const int grid = launch(false, p, {v, w, vold});
   p.q = nullptr;
   p.nq = 0;
   hipLaunchKernelGGL(k_reduce_partials<T>, dim3((k + 15) / 16), dim3(kBlock), 0, s, p.partials, grid, k, nrm2);
   PHX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_minres_rot<T>, dim3(1), dim3(64), 0, s, dots, nrm2, st, k, coef);
+  hipLaunchKernelGGL(k_minres_rot<T>, dim3(1), dim3(64), 0, s, dots, nrm2, st, k, coef, phase);
   PHX_HIP(hipGetLastError());
-  p.flags = KF_D2;
+  p.flags = KF_D2 | (phase >= 0 ? KF_NOVOLD : 0);
   p.cdev = coef;
+  p.scvold = nullptr;
   launch(true, p, {v, vold, w, d1, d2, x});
 }
 
@@ -443,10 +481,10 @@ template void krylov_minres_update<float>(int64_t, int, int64_t, const float*, f
                                           const float*);
 template void krylov_minres_step_dev<double>(int64_t, int, int64_t, const double*, double*, double*, const double*,
                                              double*, double*, const double*, MinresState*, double*, const double*,
-                                             int64_t, int, const double*, int);
+                                             int64_t, int, const double*, int, int);
 template void krylov_minres_step_dev<float>(int64_t, int, int64_t, const float*, float*, float*, const float*, float*,
                                             float*, const float*, MinresState*, float*, const float*, int64_t, int,
-                                            const float*, int);
+                                            const float*, int, int);
 template void convert_block<double, float>(const double*, int64_t, float*, int64_t, int64_t, int);
 template void convert_block<float, double>(const float*, int64_t, double*, int64_t, int64_t, int);
 

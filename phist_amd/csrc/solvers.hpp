@@ -172,13 +172,17 @@ struct Operator {
   // Y = (A + diag(s)) X with the shifts s on the device and the column dots
   // <X, Y> left on the device (dots_dev[nvec + c]; dots_dev[c] = <Y, Y>);
   // sparse matrices of this library only
-  void apply_shifted_dev(const T* s_dev, const Mvec<T>* X, Mvec<T>* Y, T* dots_dev) const {
+  // (xscale: optional per-column factors of X on the device, stride xstride)
+  void apply_shifted_dev(const T* s_dev, const Mvec<T>* X, Mvec<T>* Y, T* dots_dev, const double* xscale = nullptr,
+                         int64_t xstride = 1) const {
     PHX_REQUIRE(A != nullptr, PHIST_NOT_IMPLEMENTED, "device dots need a sparse matrix of this library");
     SpmvRequest req;
     req.has_shift = true;
     req.shifts_dev = s_dev;
     req.dot_vw = true;
     req.dots_dev = dots_dev;
+    req.xscale = xscale;
+    req.xscale_stride = xstride;
     spmv_launch<T>(A, T(1), nullptr, X, T(0), Y, req, nullptr, nullptr);
   }
 };
@@ -218,7 +222,7 @@ void copy_any(const Mvec<TS>* X, Mvec<TD>* Y) {
 template <typename T>
 void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
                             const T* dots, MinresState* st, T* work, const T* Q = nullptr, int64_t ldq = 0,
-                            int nq = 0, const T* C = nullptr, int ldc = 0);
+                            int nq = 0, const T* C = nullptr, int ldc = 0, int phase = -1);
 
 constexpr int kFusedProjCols = 64;  // projection bases up to this width are applied inside k_lanczos
 
@@ -407,6 +411,10 @@ class MinresBatch {
     if (uploads_) PHX_HIP(hipEventSynchronize(ev_[2]));  // the staging area is free again
     for (int j = j0; j < j0 + n; j++) {
       h_up()[j] = st_[j].m;
+      // v is stored normalised after a load (factor 1), v_old is zero
+      h_up()[j].sc[phase_ % 3] = 1.0;
+      h_up()[j].sc[(phase_ + 1) % 3] = 0.0;
+      h_up()[j].sc[(phase_ + 2) % 3] = 0.0;
       h_shift_[j] = st_[j].shift;
       load_seq_[j] = seq_ + 1;
       view_seq_[j] = seq_;
@@ -432,7 +440,8 @@ class MinresBatch {
     const int k = cap_;
     Mvec<T>* v = cur(V);
     Mvec<T>* w = cur(W);
-    op_.apply_shifted_dev(d_shift_, v, w, d_dots_);
+    const int ph = (int)(phase_ % 3);
+    op_.apply_shifted_dev(d_shift_, v, w, d_dots_, &d_st_[0].sc[ph], sizeof(MinresState) / sizeof(double));
     const bool fuse = Q_ && Q_->nvec <= kFusedProjCols;  // w -= Q*C inside the Lanczos update
     if (Q_) {
       gemm_gram<T>(T(1), Q_, w, T(0), C_->get(), false);
@@ -441,9 +450,18 @@ class MinresBatch {
     krylov_minres_step_dev<T>(v->nrows, k, cap_, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(),
                               cur(D2)->dev(), cur(XS)->dev(), d_dots_, d_st_, d_work_, fuse ? Q_->dev() : nullptr,
                               fuse ? Q_->lda : 0, fuse ? Q_->nvec : 0, fuse ? C_->get()->dev() : nullptr,
-                              fuse ? C_->get()->root->ld : 0);
-    std::swap(role_[V], role_[VOLD]);
+                              fuse ? C_->get()->root->ld : 0, ph);
+    advance_roles();
+  }
+  // after a device iteration: w (unnormalised) is the next v, v the next
+  // v_old, the old v_old block takes the next SpMV; the direction blocks swap
+  void advance_roles() {
+    const int b = role_[VOLD];
+    role_[VOLD] = role_[V];
+    role_[V] = role_[W];
+    role_[W] = b;
     std::swap(role_[D1], role_[D2]);
+    phase_++;
   }
 
   void drop_graphs() {
@@ -454,15 +472,16 @@ class MinresBatch {
       }
   }
 
-  // glen_ iterations captured as one HIP graph, for the role layout p the
-  // blocks are in now (the layouts alternate, so two graphs cover every
-  // start); kernel arguments name only blocks, device scalars, the
+  // glen_ iterations captured as one HIP graph, for the layout p (0..5) the
+  // rotating blocks and factors are in now (they repeat every six
+  // iterations); kernel arguments name only blocks, device scalars, the
   // projection basis and its coefficient block, and work space the batch owns
   void capture(int p) {
     Runtime& r = rt();
     hipStream_t st = r.stream;
     int saved[6];
     std::copy(role_, role_ + 6, saved);
+    const int64_t saved_phase = phase_;
     for (int attempt = 0; attempt < 2; attempt++) {
       r.work_arena = garena_;
       r.work_need = 0;
@@ -478,6 +497,7 @@ class MinresBatch {
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(st, &g);
       std::copy(saved, saved + 6, role_);
+      phase_ = saved_phase;
       if (!ok || e != hipSuccess || !g) {
         if (g) (void)hipGraphDestroy(g);
         break;
@@ -533,15 +553,12 @@ class MinresBatch {
     }
     for (int i = 0; i < n;) {
       if (graphs && !graph_off_ && n - i >= glen_) {
-        const int p = role_[V];  // 0 or 1
+        const int p = (int)(phase_ % 6);  // the layout of the rotating blocks and factors
         if (!gexec_[p]) capture(p);
         if (gexec_[p]) {
           PHX_HIP(hipGraphLaunch(gexec_[p], r.stream));
           r.graph_launches++;
-          for (int t = 0; t < glen_; t++) {
-            std::swap(role_[V], role_[VOLD]);
-            std::swap(role_[D1], role_[D2]);
-          }
+          for (int t = 0; t < glen_; t++) advance_roles();
           i += glen_;
           continue;
         }
@@ -638,7 +655,8 @@ class MinresBatch {
   std::vector<int64_t> load_seq_;  // slot j's state was uploaded before iteration load_seq_[j]
   std::vector<int64_t> view_seq_;  // the host view of slot j is its state after view_seq_[j] iterations
   // captured iterations, one per role layout, and what they were captured with
-  hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec_[6] = {};
+  int64_t phase_ = 0;  // device iterations enqueued (rotation of the blocks and factors)
   const T* gq_ = nullptr;
   int gnq_ = 0;
   int64_t gldq_ = 0;

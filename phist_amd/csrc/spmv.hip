@@ -45,6 +45,8 @@ struct SpmvArgs {
   int blocks_per_cu;  // host side: persistent grid size per CU (0 = as many as fit)
   T* __restrict__ partials;  // [gridDim.x][2][ncols]
   const T* __restrict__ shift_dev;  // shifts on the device (this launch's columns), else shift[]
+  const double* __restrict__ xscale;  // per-column factors of x (XSC kernels), stride xscale_stride
+  int64_t xscale_stride;
   T shift[64 * 4];
 };
 
@@ -98,7 +100,7 @@ __device__ inline T group_bcast_t(T v, int src, int lane) {
 // cooperatively (lane q takes slot j = jb*LPR + q, coalesced over the rows
 // of the chunk) and share them by group broadcast, so every index is read
 // from memory once.  KB loads per lane are in flight per batch.
-template <typename T, int VPL, int LPR>
+template <typename T, int VPL, int LPR, bool XSC>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   constexpr int RPW = 64 / LPR;
   constexpr int RPB = (kBlock / 64) * RPW;
@@ -114,10 +116,12 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   const int C = 1 << a.clog2;
   const int flags = a.flags;
 
-  T sh[VPL];
+  T sh[VPL], xsc[VPL];
 #pragma unroll
-  for (int e = 0; e < VPL; e++)
+  for (int e = 0; e < VPL; e++) {
     sh[e] = (colok && (flags & F_SHIFT)) ? (a.shift_dev ? a.shift_dev[c0 + e] : a.shift[c0 + e]) : T(0);
+    if constexpr (XSC) xsc[e] = colok ? (T)a.xscale[(int64_t)(c0 + e) * a.xscale_stride] : T(0);
+  }
   T dww[VPL], dvw[VPL];
 #pragma unroll
   for (int e = 0; e < VPL; e++) dww[e] = dvw[e] = T(0);
@@ -184,6 +188,13 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
       }
     }
     if (!live || !colok) continue;
+    if constexpr (XSC) {  // (A + s)(x .* f) = ((A + s) x) .* f, one product per result
+#pragma unroll
+      for (int e = 0; e < VPL; e++) {
+        acc[e] *= xsc[e];
+        xo[e] *= xsc[e];
+      }
+    }
 
     T yn[VPL];
     T* yp = a.y + (int64_t)row * a.ldy + c0;
@@ -243,9 +254,9 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 
 namespace {
 
-template <typename T, int VPL, int LPR>
+template <typename T, int VPL, int LPR, bool XSC>
 int launch_one(SpmvArgs<T>& a) {
-  auto kern = k_sell_spmv<T, VPL, LPR>;
+  auto kern = k_sell_spmv<T, VPL, LPR, XSC>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   a.ntiles = (a.nslots + RPB - 1) / RPB;
   const int grid = persistent_blocks(kern, 0, a.ntiles, a.blocks_per_cu);
@@ -258,17 +269,22 @@ int launch_one(SpmvArgs<T>& a) {
   return grid;
 }
 
-template <typename T, int VPL>
+template <typename T, int VPL, bool XSC>
 int launch_lpr(SpmvArgs<T>& a, int lpr) {
   switch (lpr) {
-    case 1: return launch_one<T, VPL, 1>(a);
-    case 2: return launch_one<T, VPL, 2>(a);
-    case 4: return launch_one<T, VPL, 4>(a);
-    case 8: return launch_one<T, VPL, 8>(a);
-    case 16: return launch_one<T, VPL, 16>(a);
-    case 32: return launch_one<T, VPL, 32>(a);
-    default: return launch_one<T, VPL, 64>(a);
+    case 1: return launch_one<T, VPL, 1, XSC>(a);
+    case 2: return launch_one<T, VPL, 2, XSC>(a);
+    case 4: return launch_one<T, VPL, 4, XSC>(a);
+    case 8: return launch_one<T, VPL, 8, XSC>(a);
+    case 16: return launch_one<T, VPL, 16, XSC>(a);
+    case 32: return launch_one<T, VPL, 32, XSC>(a);
+    default: return launch_one<T, VPL, 64, XSC>(a);
   }
+}
+
+template <typename T, int VPL>
+int launch_xsc(SpmvArgs<T>& a, int lpr) {
+  return a.xscale ? launch_lpr<T, VPL, true>(a, lpr) : launch_lpr<T, VPL, false>(a, lpr);
 }
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
@@ -333,7 +349,9 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
     const int groups = (nc + VPL - 1) / VPL;
     int lpr = 1;
     while (lpr < groups) lpr <<= 1;
-    const int grid = VPL == 1 ? launch_lpr<T, 1>(a, lpr) : launch_lpr<T, V16>(a, lpr);
+    a.xscale = req.xscale ? req.xscale + (int64_t)cb * req.xscale_stride : nullptr;
+    a.xscale_stride = req.xscale_stride;
+    const int grid = VPL == 1 ? launch_xsc<T, 1>(a, lpr) : launch_xsc<T, V16>(a, lpr);
     if (dots && req.dots_dev) {
       PHX_REQUIRE(cb == 0 && nc == nvec, PHIST_INVALID_INPUT, "device dots need one column block (%d columns)", nvec);
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
