@@ -343,6 +343,12 @@ template <typename T>
 void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C, bool to_host = true);
 template <typename T> void gemm_times_sd(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W);
 template <typename T> void gemm_times_sd_inplace(Mvec<T>* V, const SdMat<T>* M);
+// On the device, one workgroup: G (k x k, k <= 64, symmetric positive definite)
+// = R'R with R upper triangular, Rinv = R^-1 (device copy of an sdMat of the
+// same shape).  *d_flag (device) becomes 1 when a pivot is not positive or
+// below 64*eps times the largest diagonal entry of G (Rinv is then the
+// identity), else 0.
+template <typename T> void chol_inv_dev(const SdMat<T>* G, SdMat<T>* Rinv, T* d_flag);
 template <typename T>
 void gemm_times_sd_add_times_sd(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W, const SdMat<T>* D);
 template <typename T>

@@ -25,6 +25,7 @@
 #include "solvers.hpp"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -174,6 +175,12 @@ class JadaRun {
     AV_.p = api::mvec_create<T>(map, maxBase);
     Q_.p = api::mvec_create<T>(map, nEigMax);
     t_.p = api::mvec_create<T>(map, bs);
+    tback_.p = api::mvec_create<T>(map, bs);
+    oC_.p = api::sdMat_create<T>(maxBase, bs);
+    oG_.p = api::sdMat_create<T>(bs, bs);
+    oR_.p = api::sdMat_create<T>(bs, bs);
+    oF_.p = api::sdMat_create<T>(2, 1);
+    if (const char* e = getenv("PHIST_AMD_JADA_ORTH")) orth_mode_ = atoi(e);
     res_.p = api::mvec_create<T>(map, std::min(lookAhead, nEigMax));
     H_.p = api::sdMat_create<T>(maxBase, maxBase);
     solve_.resize(bs + 1);
@@ -441,21 +448,33 @@ class JadaRun {
       }
       // extend the basis: V(:, nV:nV+k) = orthonormalised t
       View<T> Vful(V_.get(), 0, nV - 1), Vv(V_.get(), nV, nV + k - 1), AVv(AV_.get(), nV, nV + k - 1);
-      SdBox<T> R1(k, k), R2(nV, k);
-      int rank = 0;
+      View<T> tb(tback_.get(), 0, k - 1);
       // orthonormalise the narrow block t (row stride bs) and copy it into
       // the basis afterwards: the Gram and in-place updates of a k-column
       // view of the wide basis (row stride maxBas) would move a whole cache
-      // line per row for k*8 useful bytes.  No rank truncation: near
-      // convergence t is tiny, and a rank test would replace it by noise
-      // (the reference passes no flags)
-      api::orthog<T>(Vful.get(), t.get(), R1.get(), R2.get(), 3, &rank, 0);
+      // line per row for k*8 useful bytes.  The device-resident variant needs
+      // no host round trip; if its Cholesky finds t (numerically) rank
+      // deficient, the robust orthogonalisation redoes the step from a copy.
+      // No rank truncation: near convergence t is tiny, and a rank test would
+      // replace it by noise (the reference passes no flags)
+      blas_copy<T>(t.get(), tb.get());
+      if (orth_mode_ != 0) orth_dev(Vful.get(), t.get(), k);
       blas_copy<T>(t.get(), Vv.get());
       // the SpMV gathers rows of its input: take them from t as well
       A.apply_shifted(T(1), nullptr, t.get(), T(0), AVv.get(), nullptr);
       View<T> Vall(V_.get(), 0, nV + k - 1);
       SdBox<T> Hn(nV + k, k);
-      gemm_gram<T>(T(1), Vall.get(), AVv.get(), T(0), Hn.get());
+      sd_from_device_async<T>(oF_.get());
+      gemm_gram<T>(T(1), Vall.get(), AVv.get(), T(0), Hn.get());  // synchronises: the flags are on the host
+      if (orth_mode_ != 1 || oF_->h(0, 0) != T(0) || oF_->h(1, 0) != T(0)) {
+        blas_copy<T>(tb.get(), t.get());
+        SdBox<T> R1(k, k), R2(nV, k);
+        int rank = 0;
+        api::orthog<T>(Vful.get(), t.get(), R1.get(), R2.get(), 3, &rank, 0);
+        blas_copy<T>(t.get(), Vv.get());
+        A.apply_shifted(T(1), nullptr, t.get(), T(0), AVv.get(), nullptr);
+        gemm_gram<T>(T(1), Vall.get(), AVv.get(), T(0), Hn.get());
+      }
       for (int j = 0; j < k; j++)
         for (int i = 0; i < nV + k; i++) {
           H(i, nV + j) = Hn->h(i, j);
@@ -510,6 +529,31 @@ class JadaRun {
   const int nEigMax;
   MvBox<T> V_, AV_, Q_, t_, res_;
   SdBox<T> H_;
+  // device-resident orthonormalisation of the correction block: a copy of
+  // the block for the robust path, coefficient and Gram work space, flags
+  MvBox<T> tback_;
+  SdBox<T> oC_, oG_, oR_, oF_;
+  // PHIST_AMD_JADA_ORTH: 1 (default) device-resident, robust path when a
+  // Cholesky fails; 0 robust path only; 2 both, the robust result kept (for
+  // tests of the fallback)
+  int orth_mode_ = 1;
+
+  // t (k columns) against the orthonormal V: classical Gram-Schmidt twice,
+  // then Cholesky QR twice, all on the device; oF_ holds one flag per
+  // Cholesky (1 = a pivot too small for t to be taken as full rank)
+  void orth_dev(const Mvec<T>* V, Mvec<T>* t, int k) {
+    const int m = V->nvec;
+    SdView<T> C(oC_.get(), 0, m - 1, 0, k - 1), G(oG_.get(), 0, k - 1, 0, k - 1), R(oR_.get(), 0, k - 1, 0, k - 1);
+    for (int pass = 0; pass < 2; pass++) {
+      gemm_gram<T>(T(1), V, t, T(0), C.get(), false);
+      gemm_times_sd<T>(T(-1), V, C.get(), T(1), t);
+    }
+    for (int pass = 0; pass < 2; pass++) {
+      gemm_gram<T>(T(1), t, t, T(0), G.get(), false);
+      chol_inv_dev<T>(G.get(), R.get(), oF_->dev() + pass);
+      gemm_times_sd_inplace<T>(t, R.get());
+    }
+  }
   // correction-equation solvers by number of systems (kept between outer iterations)
   std::vector<std::unique_ptr<MinresSolver<T, T>>> solve_;
   std::vector<std::unique_ptr<MinresSolver<T, float>>> solve32_;

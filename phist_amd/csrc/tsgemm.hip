@@ -22,6 +22,7 @@
 // depends on every load of that row.
 #include "amd_device.hpp"
 #include <algorithm>
+#include <limits>
 
 namespace phx {
 
@@ -942,7 +943,70 @@ void gemm_fused_mvsd_mvTmv(T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta,
   gemm_gram<T>(T(1), W, W, T(0), WtW);
 }
 
+// Cholesky factor of a small Gram matrix and its inverse in LDS: column j of
+// R (upper) from G(0:j, j) and the columns before it, rows of a column in
+// parallel; then each thread back-substitutes one column of R^-1
+template <typename T>
+__global__ void __launch_bounds__(64) k_chol_inv(const T* __restrict__ G, int ldg, int k, T* __restrict__ Ri, int ldr,
+                                                 T* __restrict__ flag) {
+  __shared__ T R[64][65];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  T gmax = T(0);
+  for (int j = 0; j < k; j++) {
+    if (t <= j) R[t][j] = G[(int64_t)j * ldg + t];
+    const T gjj = G[(int64_t)j * ldg + j];
+    gmax = gjj > gmax ? gjj : gmax;
+  }
+  if (t == 0) bad = 0;
+  __syncthreads();
+  const T tiny = T(64) * std::numeric_limits<T>::epsilon() * gmax;
+  for (int j = 0; j < k; j++) {
+    // pivot: R(j,j) = sqrt(G(j,j) - sum_p<j R(p,j)^2), then R(j, i) for i > j
+    if (t == 0) {
+      T d = R[j][j];
+      for (int p = 0; p < j; p++) d -= R[p][j] * R[p][j];
+      if (!(d > tiny)) bad = 1;
+      R[j][j] = d > T(0) ? sqrt(d) : T(1);
+    }
+    __syncthreads();
+    if (t > j && t < k) {
+      T s = R[j][t];
+      for (int p = 0; p < j; p++) s -= R[p][j] * R[p][t];
+      R[j][t] = s / R[j][j];
+    }
+    __syncthreads();
+  }
+  const bool fail = bad != 0;
+  if (t < k) {  // column t of R^-1 (R X(:, t) = e_t), back-substituted in place in Ri
+    T* x = Ri + (int64_t)t * ldr;
+    for (int i = k - 1; i >= 0; i--) {
+      if (i > t) {
+        x[i] = T(0);
+        continue;
+      }
+      T s = (i == t) ? T(1) : T(0);
+      for (int p = i + 1; p <= t; p++) s -= R[i][p] * x[p];
+      x[i] = s / R[i][i];
+    }
+    if (fail)
+      for (int i = 0; i < k; i++) x[i] = i == t ? T(1) : T(0);
+  }
+  if (t == 0) flag[0] = fail ? T(1) : T(0);
+}
+
+template <typename T>
+void chol_inv_dev(const SdMat<T>* G, SdMat<T>* Rinv, T* d_flag) {
+  const int k = G->nrows;
+  PHX_REQUIRE(k == G->ncols && Rinv->nrows == k && Rinv->ncols == k && k >= 1 && k <= 64, PHIST_INVALID_INPUT,
+              "device Cholesky of a %dx%d Gram matrix", G->nrows, G->ncols);
+  hipLaunchKernelGGL(k_chol_inv<T>, dim3(1), dim3(64), 0, rt().stream, G->dev(), G->root->ld, k, Rinv->dev(),
+                     Rinv->root->ld, d_flag);
+  PHX_HIP(hipGetLastError());
+}
+
 #define PHX_INST(T)                                                                                  \
+  template void chol_inv_dev<T>(const SdMat<T>*, SdMat<T>*, T*);                                     \
   template void gemm_gram<T>(T, const Mvec<T>*, const Mvec<T>*, T, SdMat<T>*, bool);                       \
   template void gemm_times_sd<T>(T, const Mvec<T>*, const SdMat<T>*, T, Mvec<T>*);                   \
   template void gemm_times_sd_inplace<T>(Mvec<T>*, const SdMat<T>*);                                 \

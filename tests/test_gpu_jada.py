@@ -199,3 +199,21 @@ def test_jada_rejects_unsupported(be64):
     assert fl.value == -99
     for o in (Q, R, A):
         o.delete()
+
+
+@pytest.mark.parametrize("mode", ["0", "2"])
+def test_jada_orthogonalisation_paths(be64, mode, monkeypatch):
+    """the robust orthogonalisation of the correction block alone (mode 0)
+    and after the device-resident one (mode 2, the fallback taken as if a
+    Cholesky had failed) reach the same eigenvalues as the default"""
+    monkeypatch.setenv("PHIST_AMD_JADA_ORTH", mode)
+    nx, ny, nz = 9, 11, 13
+    A = be64.laplace3d(nx, ny, nz)
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, 6, "SR", 1e-8, block_size=2, min_bas=14, max_bas=28,
+                                                    max_iters=400, inner_max_iters=20)
+    assert nconv >= 6, (nconv, niter, res)
+    assert np.allclose(np.sort(ev[:6]), laplace_eigs(nx, ny, nz, 6), rtol=0, atol=1e-7)
+    check_schur(be64, A, Q, R, 6, 1e-8)
+    for o in (Q, R, A):
+        o.delete()
+
