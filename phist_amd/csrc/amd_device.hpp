@@ -104,9 +104,9 @@ __device__ inline void block_tile_sum(T* sh, int nout, T* __restrict__ out, F&& 
 // in order, then the 16 group sums are added in order of p.  The result is
 // returned on threads 0..15 (others get 0); every thread must call it.
 template <typename T>
-__device__ inline T block_sum16(const T* __restrict__ partials, int nblocks, int width) {
+__device__ inline T block_sum16(const T* __restrict__ partials, int nblocks, int width, int group) {
   __shared__ T sh[16][17];
-  const int o = blockIdx.x * 16 + (threadIdx.x & 15), grp = threadIdx.x >> 4;
+  const int o = group * 16 + (threadIdx.x & 15), grp = threadIdx.x >> 4;
   T s = T(0);
   if (o < width) {
 #pragma unroll 8
@@ -118,7 +118,12 @@ __device__ inline T block_sum16(const T* __restrict__ partials, int nblocks, int
   if (threadIdx.x < 16)
 #pragma unroll
     for (int q = 0; q < kBlock / 16; q++) t += sh[q][threadIdx.x];
+  __syncthreads();  // sh may be reused by the next call
   return t;
+}
+template <typename T>
+__device__ inline T block_sum16(const T* __restrict__ partials, int nblocks, int width) {
+  return block_sum16(partials, nblocks, width, (int)blockIdx.x);
 }
 
 // Adds partials[b*width + i] over b in a fixed order; out[i] (device);

@@ -3,13 +3,17 @@
 //
 // MINRES (Paige & Saunders) on k independent shifted systems at once, one
 // column per system, the per-column scalars on the host (kernel arguments) or,
-// for krylov_minres_step_dev, on the device (k_minres_ab, k_minres_rot):
+// for krylov_minres_step_dev, on the device (column states, k_minres_rot):
 //
 //   k_lanczos    w(:,c) -= a_c * v(:,c) + b_c * vold(:,c), and the column
-//                norms of the new w (fixed-order two-stage reduction)
+//                norms of the new w (fixed-order two-stage reduction); on the
+//                device path a, b come from the column states and the SpMV dots
 //   k_minres     d_new = (v - delta_c*d1 - eps_c*d2) / gamma_c  (into d2),
-//                x += tau_c * d_new,  vold = w / beta_c  (the next Lanczos
-//                vector, written over the previous one)
+//                x += tau_c * d_new,  and on the host-scalar path
+//                vold = w / beta_c (the next Lanczos vector, written over the
+//                previous one; the device path keeps w unnormalised instead)
+//   k_minres_rot one workgroup: the norms from the k_lanczos partials, then
+//                the rotations of every column
 //   k_convert    Y = X with a change of precision (FP64 <-> FP32), used for
 //                the FP32 inner solves of the mixed-precision eigensolver
 //
@@ -50,12 +54,21 @@ struct KrylovArgs {
   const double* scv = nullptr;
   const double* scvold = nullptr;
   int64_t sc_stride = 0;
+  // optional: the Lanczos coefficients a, b taken from the MINRES column
+  // states and the fused SpMV dots (dots[ncols + c] = <v, Av>) instead of cdev
+  const MinresState* st = nullptr;
+  const T* dots = nullptr;
   T c0[kKrylovMaxCols], c1[kKrylovMaxCols], c2[kKrylovMaxCols], c3[kKrylovMaxCols], c4[kKrylovMaxCols];
 };
 
 template <typename T>
 __device__ inline T kcoef(const KrylovArgs<T>& p, int q, int c) {
   if (c >= p.ncols) return T(0);
+  if (p.st && q < 2) {  // a_c = <v, Av>_c, b_c = beta_c after the first iteration; 0 when frozen
+    const MinresState& s = p.st[c];
+    if (q == 0) return s.active ? p.dots[p.ncols + c] : T(0);
+    return (s.active && s.iters > 0) ? (T)s.beta : T(0);
+  }
   if (p.cdev) return p.cdev[q * p.ncols + c];
   switch (q) {
     case 0: return p.c0[c];
@@ -219,17 +232,6 @@ __global__ void __launch_bounds__(kBlock) k_convert(const TS* __restrict__ x, in
   }
 }
 
-// Lanczos coefficients of the active columns: a_c = <v, Av>_c (the fused SpMV
-// dots, vw half), b_c = beta_c after the first iteration; 0 for frozen columns
-template <typename T>
-__global__ void k_minres_ab(const T* __restrict__ dots, const MinresState* __restrict__ st, int k, T* __restrict__ ab) {
-  const int c = threadIdx.x;
-  if (c >= k) return;
-  const MinresState q = st[c];
-  ab[c] = q.active ? dots[k + c] : T(0);
-  ab[k + c] = (q.active && q.iters > 0) ? (T)q.beta : T(0);
-}
-
 // One step of the Paige & Saunders recurrences per active column, in FP64:
 // column `iters` of the Lanczos tridiagonal (beta_k, alpha_k, beta_{k+1})
 // meets the two previous rotations, a new rotation annihilates beta_{k+1};
@@ -237,9 +239,18 @@ __global__ void k_minres_ab(const T* __restrict__ dots, const MinresState* __res
 // and freezes columns that converged, broke down or hit their limit.
 // With phase >= 0 the factor of the next (unnormalised) Lanczos vector,
 // 1/beta_{k+1} or 0 for a column that was frozen, goes to sc[(phase + 1) % 3].
+// The squared norms of the new Lanczos vectors are first summed from the
+// k_lanczos block partials (in the order of k_reduce_partials).
 template <typename T>
-__global__ void k_minres_rot(const T* __restrict__ dots, const T* __restrict__ nrm2, MinresState* __restrict__ st,
-                             int k, T* __restrict__ coef, int phase) {
+__global__ void __launch_bounds__(kBlock) k_minres_rot(const T* __restrict__ dots, const T* __restrict__ partials,
+                                                       int nparts, MinresState* __restrict__ st, int k,
+                                                       T* __restrict__ coef, int phase) {
+  __shared__ T nrm2[kKrylovMaxCols];
+  for (int g = 0; g * 16 < k; g++) {
+    const T s = block_sum16(partials, nparts, k, g);
+    if (threadIdx.x < 16 && g * 16 + (int)threadIdx.x < k) nrm2[g * 16 + threadIdx.x] = s;
+  }
+  __syncthreads();
   const int c = threadIdx.x;
   if (c >= k) return;
   MinresState q = st[c];
@@ -418,11 +429,7 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   PHX_REQUIRE(k >= 1 && k <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns", k);
   if (nrows == 0) return;  // every column starts frozen (zero right-hand side)
   hipStream_t s = rt().stream;
-  T* ab = work;
   T* coef = work + 2 * k;
-  T* nrm2 = work + 7 * k;
-  hipLaunchKernelGGL(k_minres_ab<T>, dim3(1), dim3(64), 0, s, dots, st, k, ab);
-  PHX_HIP(hipGetLastError());
   KrylovArgs<T> p;
   p.nrows = nrows;
   p.ncols = k;
@@ -434,7 +441,9 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   p.d2 = d2;
   p.x = x;
   p.flags = KF_VOLD;
-  p.cdev = ab;
+  p.cdev = nullptr;
+  p.st = st;  // a and b from the column states and the SpMV dots (no separate launch)
+  p.dots = dots;
   p.q = Q;
   p.ldq = ldq;
   p.nq = Q ? nq : 0;
@@ -448,9 +457,8 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   const int grid = launch(false, p, {v, w, vold});
   p.q = nullptr;
   p.nq = 0;
-  hipLaunchKernelGGL(k_reduce_partials<T>, dim3((k + 15) / 16), dim3(kBlock), 0, s, p.partials, grid, k, nrm2);
-  PHX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_minres_rot<T>, dim3(1), dim3(64), 0, s, dots, nrm2, st, k, coef, phase);
+  p.st = nullptr;
+  hipLaunchKernelGGL(k_minres_rot<T>, dim3(1), dim3(kBlock), 0, s, dots, p.partials, grid, st, k, coef, phase);
   PHX_HIP(hipGetLastError());
   p.flags = KF_D2 | (phase >= 0 ? KF_NOVOLD : 0);
   p.cdev = coef;
