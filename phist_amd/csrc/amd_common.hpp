@@ -326,6 +326,8 @@ template <typename T> void blas_scale_cols(Mvec<T>* V, const T* scal_host, bool 
 template <typename T> void blas_axpby(const T* alpha_host, bool alpha_uniform, const Mvec<T>* X, T beta, Mvec<T>* Y);
 template <typename T> void blas_elemwise(T alpha, const Mvec<T>* V, Mvec<T>* W);
 template <typename T> void blas_dot(const Mvec<T>* V, const Mvec<T>* W, T* out_host);
+// column dots left on the device (out_dev[0..nvec)); no host synchronisation
+template <typename T> void blas_dot_dev(const Mvec<T>* V, const Mvec<T>* W, T* out_dev);
 template <typename T> void blas_copy(const Mvec<T>* X, Mvec<T>* Y);
 
 // CARP: multicoloured Kaczmarz sweeps (carp.hip)

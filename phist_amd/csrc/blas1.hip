@@ -170,10 +170,12 @@ int launch_vpl(Kind kind, RowArgs<T>& p, int lpr) {
 }
 
 // runs kind over the columns of Y (and X) in blocks of at most 64*VPL;
-// per-column coefficients come from fa/fb, dots are written to out_host
+// per-column coefficients come from fa/fb, dots are written to out_host (or,
+// when out_dev is set, left on the device in out_dev)
 template <typename T, class FA, class FB>
 void run_rows(Kind kind, const T* xdev, int64_t ldx, T* ydev, int64_t ldy, int64_t nrows, int ncols, T alpha, int flags,
-              FA fa, FB fb, T* out_host) {
+              FA fa, FB fb, T* out_host, T* out_dev = nullptr) {
+  PHX_REQUIRE(!(kind == K_DOT && out_dev && nrows == 0), PHIST_INVALID_INPUT, "device dots of an empty block");
   if (nrows == 0 || ncols == 0) {
     if (kind == K_DOT)
       for (int c = 0; c < ncols; c++) out_host[c] = T(0);
@@ -203,7 +205,13 @@ void run_rows(Kind kind, const T* xdev, int64_t ldx, T* ydev, int64_t ldy, int64
     int lpr = 1;
     while (lpr < groups) lpr <<= 1;
     const int grid = VPL == 1 ? launch_vpl<T, 1>(kind, p, lpr) : launch_vpl<T, V16>(kind, p, lpr);
-    if (kind == K_DOT) reduce_partials_to_host(p.partials, grid, nc, out_host + cb);
+    if (kind == K_DOT && out_dev) {
+      hipLaunchKernelGGL(k_reduce_partials<T>, dim3((nc + 15) / 16), dim3(kBlock), 0, rt().stream, p.partials, grid, nc,
+                         out_dev + cb);
+      PHX_HIP(hipGetLastError());
+    } else if (kind == K_DOT) {
+      reduce_partials_to_host(p.partials, grid, nc, out_host + cb);
+    }
   }
 }
 
@@ -274,7 +282,16 @@ void blas_dot(const Mvec<T>* V, const Mvec<T>* W, T* out_host) {
               zero_fn<T>, out_host);
 }
 
+template <typename T>
+void blas_dot_dev(const Mvec<T>* V, const Mvec<T>* W, T* out_dev) {
+  PHX_REQUIRE(V->nvec == W->nvec && V->nrows == W->nrows && V->nrows > 0, PHIST_INVALID_INPUT,
+              "block shapes differ or are empty");
+  run_rows<T>(K_DOT, V->dev(), V->lda, const_cast<T*>(W->dev()), W->lda, W->nrows, W->nvec, T(0), 0, zero_fn<T>,
+              zero_fn<T>, nullptr, out_dev);
+}
+
 #define PHX_INST(T)                                                           \
+  template void blas_dot_dev<T>(const Mvec<T>*, const Mvec<T>*, T*);          \
   template void blas_put_value<T>(Mvec<T>*, T);                               \
   template void blas_scale_cols<T>(Mvec<T>*, const T*, bool);                 \
   template void blas_axpby<T>(const T*, bool, const Mvec<T>*, T, Mvec<T>*);   \

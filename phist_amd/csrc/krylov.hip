@@ -294,6 +294,34 @@ __global__ void __launch_bounds__(kBlock) k_minres_rot(const T* __restrict__ dot
   coef[4 * k + c] = ib;
 }
 
+// Column states j0 .. j0+n-1 of freshly loaded systems from the squared norms
+// of their right-hand sides (nrm2[0..n)): beta1 = phibar = ||b||, active when
+// ||b|| > 0 and iterations are allowed, and the factor 1/||b|| of the
+// unnormalised first Lanczos vector in sc[ph].
+template <typename T>
+__global__ void k_minres_init(MinresState* __restrict__ st, const T* __restrict__ nrm2, int j0, int n, int ph) {
+  const int c = threadIdx.x;
+  if (c >= n) return;
+  MinresState q = st[j0 + c];
+  const T nt = sqrt(nrm2[c]);
+  const double nn = (double)nt;
+  q.beta1 = nn;
+  q.phibar = nn;
+  q.relres = nn > 0.0 ? 1.0 : 0.0;
+  q.active = (nn > 0.0 && q.max_iter > 0) ? 1 : 0;
+  q.sc[ph] = nn > 0.0 ? 1.0 / nn : 0.0;
+  st[j0 + c] = q;
+}
+
+template <typename T>
+void krylov_minres_init_dev(MinresState* st, const T* nrm2, int j0, int n, int ph) {
+  PHX_REQUIRE(n >= 1 && n <= kKrylovMaxCols && ph >= 0 && ph < 3, PHIST_INVALID_INPUT, "MINRES init of %d columns", n);
+  hipLaunchKernelGGL(k_minres_init<T>, dim3(1), dim3(64), 0, rt().stream, st, nrm2, j0, n, ph);
+  PHX_HIP(hipGetLastError());
+}
+template void krylov_minres_init_dev<double>(MinresState*, const double*, int, int, int);
+template void krylov_minres_init_dev<float>(MinresState*, const float*, int, int, int);
+
 namespace {
 
 inline bool al16(const void* ptr) { return ((uintptr_t)ptr & 15) == 0; }

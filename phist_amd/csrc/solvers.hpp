@@ -224,6 +224,10 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
                             const T* dots, MinresState* st, T* work, const T* Q = nullptr, int64_t ldq = 0,
                             int nq = 0, const T* C = nullptr, int ldc = 0, int phase = -1);
 
+// device column states of freshly loaded systems (see krylov.hip)
+template <typename T>
+void krylov_minres_init_dev(MinresState* st, const T* nrm2, int j0, int n, int ph);
+
 constexpr int kFusedProjCols = 64;  // projection bases up to this width are applied inside k_lanczos
 
 template <typename T>
@@ -308,6 +312,17 @@ class MinresBatch {
     const int k = B->nvec;
     PHX_REQUIRE(k == cap_, PHIST_INVALID_INPUT, "load_all needs %d columns", cap_);
     drain();
+    if (dev_ && B->nrows > 0) {
+      // no host round trip: b stays unnormalised in v (factor 1/||b|| in the
+      // column state), the norms go from the device dots into the states
+      for (int r : {VOLD, D1, D2, XS}) blas_put_value<T>(cur(r), T(0));
+      copy_any<TS, T>(B, cur(V));
+      blas_dot_dev<T>(cur(V), cur(V), d_work_);
+      for (int j = 0; j < k; j++) start(j, 1.0, shifts[j], tol[j], max_iter[j]);  // the norms follow on the device
+      upload(0, k);
+      krylov_minres_init_dev<T>(d_st_, d_work_, 0, k, (int)(phase_ % 3));
+      return;
+    }
     for (int b = 0; b < 6; b++) blas_put_value<T>(blk_[b].p, T(0));
     Mvec<T>* v = cur(V);
     copy_any<TS, T>(B, v);
@@ -325,6 +340,20 @@ class MinresBatch {
   // slot j from column c of B (x = 0)
   template <typename TS>
   void load(int j, const Mvec<TS>* B, int c, T shift, double tol, int max_iter) {
+    if (dev_ && B->nrows > 0) {  // as load_all: no host round trip
+      for (int r : {VOLD, D1, D2, XS}) {
+        View<T> col(cur(r), j, j);
+        blas_put_value<T>(col.get(), T(0));
+      }
+      View<T> vj(cur(V), j, j);
+      View<TS> bc(B, c, c);
+      copy_any<TS, T>(bc.get(), vj.get());
+      blas_dot_dev<T>(vj.get(), vj.get(), d_work_ + j);
+      start(j, 1.0, shift, tol, max_iter);
+      upload(j, 1);
+      krylov_minres_init_dev<T>(d_st_, d_work_ + j, j, 1, (int)(phase_ % 3));
+      return;
+    }
     for (int b = 0; b < 6; b++) {
       View<T> col(blk_[b].p, j, j);
       blas_put_value<T>(col.get(), T(0));
