@@ -328,6 +328,8 @@ template <typename T> void blas_elemwise(T alpha, const Mvec<T>* V, Mvec<T>* W);
 template <typename T> void blas_dot(const Mvec<T>* V, const Mvec<T>* W, T* out_host);
 // column dots left on the device (out_dev[0..nvec)); no host synchronisation
 template <typename T> void blas_dot_dev(const Mvec<T>* V, const Mvec<T>* W, T* out_dev);
+// V(:, c) *= 1/sqrt(n2_dev[c]) (0 where n2_dev[c] <= 0), squared norms on the device
+template <typename T> void blas_scale_inv_norm_dev(Mvec<T>* V, const T* n2_dev);
 template <typename T> void blas_copy(const Mvec<T>* X, Mvec<T>* Y);
 
 // CARP: multicoloured Kaczmarz sweeps (carp.hip)

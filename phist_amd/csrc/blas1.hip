@@ -282,6 +282,31 @@ void blas_dot(const Mvec<T>* V, const Mvec<T>* W, T* out_host) {
               zero_fn<T>, out_host);
 }
 
+// V(:, c) *= (T)(1 / sqrt(n2[c])) with the squared norms on the device (0
+// when n2[c] is not positive): a grid-stride loop over the elements
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_scale_inv_norm(T* __restrict__ v, int64_t ld, int64_t nrows, int ncols,
+                                                           const T* __restrict__ n2) {
+  const int64_t n = nrows * ncols;
+  for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = e / ncols;
+    const int c = (int)(e - i * ncols);
+    const double q = (double)n2[c];
+    const T s = q > 0.0 ? (T)(1.0 / sqrt(q)) : T(0);
+    v[i * ld + c] *= s;
+  }
+}
+
+template <typename T>
+void blas_scale_inv_norm_dev(Mvec<T>* V, const T* n2_dev) {
+  const int64_t n = V->nrows * V->nvec;
+  if (n == 0) return;
+  const int grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, (int64_t)rt().num_cu * 8);
+  hipLaunchKernelGGL(k_scale_inv_norm<T>, dim3(std::max(grid, 1)), dim3(kBlock), 0, rt().stream, V->dev(), V->lda,
+                     V->nrows, V->nvec, n2_dev);
+  PHX_HIP(hipGetLastError());
+}
+
 template <typename T>
 void blas_dot_dev(const Mvec<T>* V, const Mvec<T>* W, T* out_dev) {
   PHX_REQUIRE(V->nvec == W->nvec && V->nrows == W->nrows && V->nrows > 0, PHIST_INVALID_INPUT,
@@ -292,6 +317,7 @@ void blas_dot_dev(const Mvec<T>* V, const Mvec<T>* W, T* out_dev) {
 
 #define PHX_INST(T)                                                           \
   template void blas_dot_dev<T>(const Mvec<T>*, const Mvec<T>*, T*);          \
+  template void blas_scale_inv_norm_dev<T>(Mvec<T>*, const T*);               \
   template void blas_put_value<T>(Mvec<T>*, T);                               \
   template void blas_scale_cols<T>(Mvec<T>*, const T*, bool);                 \
   template void blas_axpby<T>(const T*, bool, const Mvec<T>*, T, Mvec<T>*);   \

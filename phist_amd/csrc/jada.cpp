@@ -231,57 +231,107 @@ class JadaRun {
       // basis would cost a whole cache line per row for 8 useful bytes; each
       // is copied into the basis once
       MvBox<T> wcol(map, 1), acol(map, 1), ncol(map, 1);
-      SdBox<T> hc(maxBase, 2);  // the two Gram-Schmidt passes' coefficients (device, then host)
+      // first pass without host round trips: every step's Gram-Schmidt
+      // coefficients and squared norm stay on the device until the end; a
+      // step whose A v_j lies numerically in the basis sends the whole
+      // start-up through the step-by-step robust variant below
+      bool fast_ok = true;
       {
-        View<T> v0(V_.get(), 0, 0);
-        blas_copy<T>(v0.get(), wcol.get());
-      }
-      for (int j = 0; j < nV; j++) {
-        View<T> avj(AV_.get(), j, j);
-        A.apply_shifted(T(1), nullptr, wcol.get(), T(0), acol.get(), nullptr);
-        blas_copy<T>(acol.get(), avj.get());
-        if (j + 1 >= maxBase) break;
-        View<T> Vj(V_.get(), 0, j), vn(V_.get(), j + 1, j + 1);
-        blas_copy<T>(acol.get(), ncol.get());
-        // classical Gram-Schmidt twice against the orthonormal V(:, 0:j), the
-        // coefficients kept on the device: one host round trip per step (the
-        // norm), where the robust orthogonalisation takes several
-        SdView<T> h1(hc.get(), 0, j, 0, 0), h2(hc.get(), 0, j, 1, 1), hh(hc.get(), 0, j, 0, 1);
-        gemm_gram<T>(T(1), Vj.get(), ncol.get(), T(0), h1.get(), false);
-        gemm_times_sd<T>(T(-1), Vj.get(), h1.get(), T(1), ncol.get());
-        gemm_gram<T>(T(1), Vj.get(), ncol.get(), T(0), h2.get(), false);
-        gemm_times_sd<T>(T(-1), Vj.get(), h2.get(), T(1), ncol.get());
-        sd_from_device_async<T>(hh.get());
-        T n2;
-        blas_dot<T>(ncol.get(), ncol.get(), &n2);  // synchronises: hh is on the host too
-        double hn2 = 0.0;
-        for (int i = 0; i <= j; i++) {
-          const double h = (double)hh->h(i, 0) + (double)hh->h(i, 1);
-          hn2 += h * h;
+        SdBox<T> hall(maxBase, 2 * maxBase), n2all(maxBase, 1);
+        {
+          View<T> v0(V_.get(), 0, 0);
+          blas_copy<T>(v0.get(), wcol.get());
         }
-        const double nrm = std::sqrt((double)n2);
-        if (nrm > 1e-12 * std::sqrt(hn2 + (double)n2) && std::isfinite(nrm)) {
-          const T sc = (T)(1.0 / nrm);
-          blas_scale_cols<T>(ncol.get(), &sc, true);
-          for (int i = 0; i <= j; i++) H(i, j) = hh->h(i, 0) + hh->h(i, 1);
-          H(j + 1, j) = (T)nrm;
-        } else {
-          // A v_j (numerically) in span(V): the robust orthogonalisation continues
-          // with a random orthogonal vector (phist_simple_arnoldi_def.hpp:118-141)
+        int nsteps = 0;
+        for (int j = 0; j < nV; j++) {
+          View<T> avj(AV_.get(), j, j);
+          A.apply_shifted(T(1), nullptr, wcol.get(), T(0), acol.get(), nullptr);
+          blas_copy<T>(acol.get(), avj.get());
+          if (j + 1 >= maxBase) break;
+          View<T> Vj(V_.get(), 0, j), vn(V_.get(), j + 1, j + 1);
           blas_copy<T>(acol.get(), ncol.get());
-          SdBox<T> R1(1, 1), R2(j + 1, 1);
-          int rank = 0;
-          const int dim0 = api::orthog<T>(Vj.get(), ncol.get(), R1.get(), R2.get(), 3, &rank, 131072);
-          for (int i = 0; i <= j; i++) H(i, j) = R2->h(i, 0);
-          H(j + 1, j) = R1->h(0, 0);
-          if (dim0 == 0) {  // a more accurate subdiagonal entry: v_{j+1}' A v_j
-            T h;
-            blas_dot<T>(ncol.get(), acol.get(), &h);
-            H(j + 1, j) = h;
-          }
+          // classical Gram-Schmidt twice against the orthonormal V(:, 0:j)
+          SdView<T> h1(hall.get(), 0, j, 2 * j, 2 * j), h2(hall.get(), 0, j, 2 * j + 1, 2 * j + 1);
+          gemm_gram<T>(T(1), Vj.get(), ncol.get(), T(0), h1.get(), false);
+          gemm_times_sd<T>(T(-1), Vj.get(), h1.get(), T(1), ncol.get());
+          gemm_gram<T>(T(1), Vj.get(), ncol.get(), T(0), h2.get(), false);
+          gemm_times_sd<T>(T(-1), Vj.get(), h2.get(), T(1), ncol.get());
+          blas_dot_dev<T>(ncol.get(), ncol.get(), n2all->dev() + j);
+          blas_scale_inv_norm_dev<T>(ncol.get(), n2all->dev() + j);
+          blas_copy<T>(ncol.get(), vn.get());
+          std::swap(wcol.p, ncol.p);  // v_{j+1} is the next input
+          nsteps = j + 1;
         }
-        blas_copy<T>(ncol.get(), vn.get());
-        std::swap(wcol.p, ncol.p);  // v_{j+1} is the next input
+        sd_from_device_async<T>(hall.get());
+        sd_from_device<T>(n2all.get());  // one synchronisation for the whole start-up
+        for (int j = 0; j < nsteps && fast_ok; j++) {
+          double hn2 = 0.0;
+          for (int i = 0; i <= j; i++) {
+            const double h = (double)hall->h(i, 2 * j) + (double)hall->h(i, 2 * j + 1);
+            hn2 += h * h;
+          }
+          const double n2 = (double)n2all->h(j, 0), nrm = std::sqrt(n2);
+          if (!(nrm > 1e-12 * std::sqrt(hn2 + n2)) || !std::isfinite(nrm)) {
+            fast_ok = false;
+            break;
+          }
+          for (int i = 0; i <= j; i++) H(i, j) = hall->h(i, 2 * j) + hall->h(i, 2 * j + 1);
+          H(j + 1, j) = (T)nrm;
+        }
+      }
+      if (!fast_ok) {
+        SdBox<T> hc(maxBase, 2);  // the two Gram-Schmidt passes' coefficients (device, then host)
+        {
+          View<T> v0(V_.get(), 0, 0);
+          blas_copy<T>(v0.get(), wcol.get());
+        }
+        for (int j = 0; j < nV; j++) {
+          View<T> avj(AV_.get(), j, j);
+          A.apply_shifted(T(1), nullptr, wcol.get(), T(0), acol.get(), nullptr);
+          blas_copy<T>(acol.get(), avj.get());
+          if (j + 1 >= maxBase) break;
+          View<T> Vj(V_.get(), 0, j), vn(V_.get(), j + 1, j + 1);
+          blas_copy<T>(acol.get(), ncol.get());
+          // classical Gram-Schmidt twice against the orthonormal V(:, 0:j), the
+          // coefficients kept on the device: one host round trip per step (the
+          // norm), and the robust orthogonalisation where the norm is too small
+          SdView<T> h1(hc.get(), 0, j, 0, 0), h2(hc.get(), 0, j, 1, 1), hh(hc.get(), 0, j, 0, 1);
+          gemm_gram<T>(T(1), Vj.get(), ncol.get(), T(0), h1.get(), false);
+          gemm_times_sd<T>(T(-1), Vj.get(), h1.get(), T(1), ncol.get());
+          gemm_gram<T>(T(1), Vj.get(), ncol.get(), T(0), h2.get(), false);
+          gemm_times_sd<T>(T(-1), Vj.get(), h2.get(), T(1), ncol.get());
+          sd_from_device_async<T>(hh.get());
+          T n2;
+          blas_dot<T>(ncol.get(), ncol.get(), &n2);  // synchronises: hh is on the host too
+          double hn2 = 0.0;
+          for (int i = 0; i <= j; i++) {
+            const double h = (double)hh->h(i, 0) + (double)hh->h(i, 1);
+            hn2 += h * h;
+          }
+          const double nrm = std::sqrt((double)n2);
+          if (nrm > 1e-12 * std::sqrt(hn2 + (double)n2) && std::isfinite(nrm)) {
+            const T sc = (T)(1.0 / nrm);
+            blas_scale_cols<T>(ncol.get(), &sc, true);
+            for (int i = 0; i <= j; i++) H(i, j) = hh->h(i, 0) + hh->h(i, 1);
+            H(j + 1, j) = (T)nrm;
+          } else {
+            // A v_j (numerically) in span(V): the robust orthogonalisation continues
+            // with a random orthogonal vector (phist_simple_arnoldi_def.hpp:118-141)
+            blas_copy<T>(acol.get(), ncol.get());
+            SdBox<T> R1(1, 1), R2(j + 1, 1);
+            int rank = 0;
+            const int dim0 = api::orthog<T>(Vj.get(), ncol.get(), R1.get(), R2.get(), 3, &rank, 131072);
+            for (int i = 0; i <= j; i++) H(i, j) = R2->h(i, 0);
+            H(j + 1, j) = R1->h(0, 0);
+            if (dim0 == 0) {  // a more accurate subdiagonal entry: v_{j+1}' A v_j
+              T h;
+              blas_dot<T>(ncol.get(), acol.get(), &h);
+              H(j + 1, j) = h;
+            }
+          }
+          blas_copy<T>(ncol.get(), vn.get());
+          std::swap(wcol.p, ncol.p);  // v_{j+1} is the next input
+        }
       }
       // H = V'AV over the first nV columns (the square part of the Arnoldi
       // Hessenberg); symmetrise exactly

@@ -217,3 +217,27 @@ def test_jada_orthogonalisation_paths(be64, mode, monkeypatch):
     for o in (Q, R, A):
         o.delete()
 
+
+
+def test_jada_invariant_start_space(be64, tmp_path):
+    """a diagonal matrix with three distinct values: the Krylov space of the
+    start vector is invariant after three steps, so the start-up Arnoldi must
+    continue with random orthogonal vectors (the step-by-step robust path)"""
+    n = 600
+    d = np.array([1.0, 2.0, 3.0])[np.arange(n) % 3]
+    f = tmp_path / "diag3.mm"
+    with open(f, "w") as fh:
+        fh.write("%%MatrixMarket matrix coordinate real general\n")
+        fh.write(f"{n} {n} {n}\n")
+        for i in range(n):
+            fh.write(f"{i + 1} {i + 1} {d[i]:.1f}\n")
+    A = be64.read_mm(str(f))
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, 2, "SR", 1e-8, block_size=1, min_bas=8, max_bas=16,
+                                                    max_iters=200)
+    assert nconv >= 2, (nconv, niter, res)
+    assert np.allclose(ev[:2], 1.0, atol=1e-7), ev[:2]
+    Qh = Q.get()[:, :2]
+    assert np.allclose(Qh.T @ Qh, np.eye(2), atol=1e-10)
+    assert np.allclose(d[:, None] * Qh, Qh * 1.0, atol=1e-7)
+    for o in (Q, R, A):
+        o.delete()
