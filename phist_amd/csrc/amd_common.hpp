@@ -353,6 +353,8 @@ template <typename T> void gemm_times_sd_inplace(Mvec<T>* V, const SdMat<T>* M);
 // below 64*eps times the largest diagonal entry of G (Rinv is then the
 // identity), else 0.
 template <typename T> void chol_inv_dev(const SdMat<T>* G, SdMat<T>* Rinv, T* d_flag);
+// W -= V*C and D = V'W (new W) in one sweep where the shapes allow; D on the device only
+template <typename T> void gemm_sub_gram_dev(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W, SdMat<T>* D);
 template <typename T>
 void gemm_times_sd_add_times_sd(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W, const SdMat<T>* D);
 template <typename T>

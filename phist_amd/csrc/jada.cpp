@@ -177,6 +177,7 @@ class JadaRun {
     t_.p = api::mvec_create<T>(map, bs);
     tback_.p = api::mvec_create<T>(map, bs);
     oC_.p = api::sdMat_create<T>(maxBase, bs);
+    oC2_.p = api::sdMat_create<T>(maxBase, bs);
     oG_.p = api::sdMat_create<T>(bs, bs);
     oR_.p = api::sdMat_create<T>(bs, bs);
     oF_.p = api::sdMat_create<T>(2, 1);
@@ -582,7 +583,7 @@ class JadaRun {
   // device-resident orthonormalisation of the correction block: a copy of
   // the block for the robust path, coefficient and Gram work space, flags
   MvBox<T> tback_;
-  SdBox<T> oC_, oG_, oR_, oF_;
+  SdBox<T> oC_, oC2_, oG_, oR_, oF_;
   // PHIST_AMD_JADA_ORTH: 1 (default) device-resident, robust path when a
   // Cholesky fails; 0 robust path only; 2 both, the robust result kept (for
   // tests of the fallback)
@@ -593,11 +594,12 @@ class JadaRun {
   // Cholesky (1 = a pivot too small for t to be taken as full rank)
   void orth_dev(const Mvec<T>* V, Mvec<T>* t, int k) {
     const int m = V->nvec;
-    SdView<T> C(oC_.get(), 0, m - 1, 0, k - 1), G(oG_.get(), 0, k - 1, 0, k - 1), R(oR_.get(), 0, k - 1, 0, k - 1);
-    for (int pass = 0; pass < 2; pass++) {
-      gemm_gram<T>(T(1), V, t, T(0), C.get(), false);
-      gemm_times_sd<T>(T(-1), V, C.get(), T(1), t);
-    }
+    SdView<T> C(oC_.get(), 0, m - 1, 0, k - 1), C2(oC2_.get(), 0, m - 1, 0, k - 1), G(oG_.get(), 0, k - 1, 0, k - 1),
+        R(oR_.get(), 0, k - 1, 0, k - 1);
+    // the first pass's update and the second pass's Gram share one sweep over V
+    gemm_gram<T>(T(1), V, t, T(0), C.get(), false);
+    gemm_sub_gram_dev<T>(V, C.get(), t, C2.get());
+    gemm_times_sd<T>(T(-1), V, C2.get(), T(1), t);
     for (int pass = 0; pass < 2; pass++) {
       gemm_gram<T>(T(1), t, t, T(0), G.get(), false);
       chol_inv_dev<T>(G.get(), R.get(), oF_->dev() + pass);

@@ -738,7 +738,9 @@ template <typename T, int KS, int TM>
 int mix_gram_tv(MixArgs<T>& p, bool self) {
   if (self) return mix_gram_launch<T, KS, TM, 1, 1>(p);
   if (p.ngv <= 16) return mix_gram_launch<T, KS, TM, 2, 1>(p);
-  return mix_gram_launch<T, KS, TM, 2, 2>(p);
+  if (p.ngv <= 32) return mix_gram_launch<T, KS, TM, 2, 2>(p);
+  if (p.ngv <= 48) return mix_gram_launch<T, KS, TM, 2, 3>(p);
+  return mix_gram_launch<T, KS, TM, 2, 4>(p);
 }
 
 template <typename T, int KS>
@@ -813,10 +815,11 @@ bool mix_fits(int k, int m, bool with_d) {
 // (8 <= k <= 64, 8 <= m <= 32, G at most 32 columns, G not overlapping W).
 template <typename T>
 bool mix_gram(int mode, T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mvec<T>* W, const Mvec<T>* G, T ga,
-              T gb, SdMat<T>* D) {
+              T gb, SdMat<T>* D, bool to_host = true, bool narrow = false) {
+  // narrow: also 1..7 output columns and a Gram side G of up to 64 columns
   const int k = V->nvec, m = C->ncols;
-  if (!rt().mix_mfma || k < 8 || k > 64 || m < 8 || m > 32) return false;
-  if (G && (G->nvec > 32 || G->nrows != W->nrows)) return false;
+  if (!rt().mix_mfma || k < 8 || k > 64 || m < (narrow ? 1 : 8) || m > 32) return false;
+  if (G && (G->nvec > (narrow ? 64 : 32) || G->nrows != W->nrows)) return false;
   if (G && G->root == W->root && !(G->jofs + G->nvec <= W->jofs || W->jofs + m <= G->jofs)) return false;
   constexpr int V16 = 16 / sizeof(T);
   MixArgs<T> p{};
@@ -846,7 +849,7 @@ bool mix_gram(int mode, T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mv
   hipLaunchKernelGGL(k_gram_finalize<T>, dim3((nout + 15) / 16), dim3(kBlock), 0, rt().stream, p.partials, nparts, na,
                      m, false, ga, gb, D->dev(), D->root->ld);
   PHX_HIP(hipGetLastError());
-  sd_from_device(D);
+  if (to_host) sd_from_device(D);
   return true;
 }
 
@@ -1005,8 +1008,25 @@ void chol_inv_dev(const SdMat<T>* G, SdMat<T>* Rinv, T* d_flag) {
   PHX_HIP(hipGetLastError());
 }
 
+// W -= V*C, then D = V'W with the new W, in one sweep over V where the
+// shapes allow (the Gram reads the V rows the row mix has just loaded); D
+// stays on the device
+template <typename T>
+void gemm_sub_gram_dev(const Mvec<T>* V, const SdMat<T>* C, Mvec<T>* W, SdMat<T>* D) {
+  PHX_REQUIRE(V->nrows == W->nrows && C->nrows == V->nvec && C->ncols == W->nvec && D->nrows == V->nvec &&
+                  D->ncols == W->nvec,
+              PHIST_INVALID_INPUT, "shapes do not match");
+  if (V->root == W->root)
+    PHX_REQUIRE(V->jofs + V->nvec <= W->jofs || W->jofs + W->nvec <= V->jofs, PHIST_INVALID_INPUT,
+                "V and W overlap in memory");
+  if (mix_gram<T>(0, T(-1), V, C, T(1), W, V, T(1), T(0), D, false, true)) return;
+  gemm_times_sd<T>(T(-1), V, C, T(1), W);
+  gemm_gram<T>(T(1), V, W, T(0), D, false);
+}
+
 #define PHX_INST(T)                                                                                  \
   template void chol_inv_dev<T>(const SdMat<T>*, SdMat<T>*, T*);                                     \
+  template void gemm_sub_gram_dev<T>(const Mvec<T>*, const SdMat<T>*, Mvec<T>*, SdMat<T>*);          \
   template void gemm_gram<T>(T, const Mvec<T>*, const Mvec<T>*, T, SdMat<T>*, bool);                       \
   template void gemm_times_sd<T>(T, const Mvec<T>*, const SdMat<T>*, T, Mvec<T>*);                   \
   template void gemm_times_sd_inplace<T>(Mvec<T>*, const SdMat<T>*);                                 \
