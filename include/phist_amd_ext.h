@@ -29,9 +29,9 @@ void phist_amd_set_grid_blocks(int nblocks, int* iflag);
  * L2-resident, 3 for such rows and narrower blocks, 6 for shorter rows) */
 void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
 /* the device-resident MINRES iteration (blocked MINRES, the eigensolver's
- * correction equations, the service loop): iters_per_call (>= 1, default 1)
- * iterations are queued per host step, never more than an unfinished system
- * may still take; enable != 0 (default 0) replays runs of graph_len (>= 1,
+ * correction equations, the service loop): iters_per_call iterations are
+ * queued per host step, never more than an unfinished system may still take
+ * (0, the default: 2 for blocks of at most 4M elements, else 1); enable != 0 (default 0) replays runs of graph_len (>= 1,
  * default 4) iterations from captured HIP graphs (one per layout of the
  * rotating blocks, captured on first use and again when the projection
  * basis changes) instead of launching their kernels one by one.  Results are

@@ -296,7 +296,7 @@ class Backend:
             raise PhistError("blockedMINRES_shifted", fl.value)
         return it, rr, fl.value
 
-    def set_minres_graphs(self, enable, iters_per_call=1, graph_len=4):
+    def set_minres_graphs(self, enable, iters_per_call=0, graph_len=4):
         """device MINRES: queue up to iters_per_call iterations per step; with enable, replay runs of
         graph_len iterations from captured HIP graphs"""
         self.gcall("amd_set_minres_graphs", ctypes.c_int(1 if enable else 0), ctypes.c_int(iters_per_call),

@@ -117,7 +117,7 @@ struct Runtime {
   // minres_graph_len iterations are replayed from captured HIP graphs instead
   // of launching their kernels one by one (measured no faster on MI355X for
   // 3k..262k rows, so off by default)
-  int minres_iters_per_call = 1;
+  int minres_iters_per_call = 0;  // 0: 2 for blocks of <= 4M elements (host-bound), else 1
   bool minres_graphs = false;
   int minres_graph_len = 4;
   int64_t graph_captures = 0, graph_launches = 0;

@@ -57,7 +57,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_MINRES_DEVICE")) r.minres_device = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_ROWS")) r.gram_rows = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH")) r.minres_graphs = atoi(s) != 0;
-  if (const char* s = getenv("PHIST_AMD_MINRES_ITERS_PER_CALL")) r.minres_iters_per_call = std::max(1, atoi(s));
+  if (const char* s = getenv("PHIST_AMD_MINRES_ITERS_PER_CALL")) r.minres_iters_per_call = std::max(0, atoi(s));
   if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH_LEN")) r.minres_graph_len = std::max(1, atoi(s));
   kiss_reset();
   r.initialized = true;
@@ -531,7 +531,7 @@ void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag) {
 void phist_amd_set_minres_graphs(int enable, int iters_per_call, int graph_len, int* iflag) {
   guard(iflag, __func__, [&] {
     ensure_init();
-    PHX_REQUIRE(iters_per_call >= 1 && graph_len >= 1, PHIST_INVALID_INPUT, "%d iterations per call, %d per graph",
+    PHX_REQUIRE(iters_per_call >= 0 && graph_len >= 1, PHIST_INVALID_INPUT, "%d iterations per call, %d per graph",
                 iters_per_call, graph_len);
     rt().minres_graphs = enable != 0;
     rt().minres_iters_per_call = iters_per_call;

@@ -562,7 +562,11 @@ class MinresBatch {
       drain();
       return;
     }
-    const int n = (int)std::max<int64_t>(1, std::min<int64_t>(r.minres_iters_per_call, rem));
+    // small blocks are bound by the host's launches and round trips: two
+    // iterations per step there (measured marginally faster on a 64^3 JD)
+    const int ipc = r.minres_iters_per_call > 0 ? r.minres_iters_per_call
+                                                : (cur(V)->nrows * (int64_t)cap_ <= (int64_t(1) << 22) ? 2 : 1);
+    const int n = (int)std::max<int64_t>(1, std::min<int64_t>(ipc, rem));
     const bool graphs = r.minres_graphs && !graph_off_ && !r.ktimer_on && cur(V)->nrows > 0 &&
                         (!Q_ || Q_->nvec <= kFusedProjCols);
     if (graphs) {  // the graphs hold the projection's pointers and width

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 def graphs(be64):
     """set_minres_graphs(enable, iters_per_call, graph_len); the defaults come back afterwards"""
     yield be64.set_minres_graphs
-    be64.set_minres_graphs(False, 1, 4)
+    be64.set_minres_graphs(False, 0, 4)
 
 
 def minres(be, A, shifts, Bh, tol, maxit, fp32):
