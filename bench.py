@@ -294,6 +294,49 @@ def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
     A.delete()
 
 
+def gmres_bench(be, args, n=128, nsys=16, tol=1e-8, restart=30, max_iter=600):
+    """Restarted blocked GMRES on shifted systems: 3D Laplacian 128^3 (2,097,152 rows), 16 shifts in
+    [0.05, 1.0], random right-hand sides, zero initial guess, GMRES(30) to tol 1e-8, FP64 and FP32;
+    reports the time per Arnoldi step (the Krylov basis of a step holds nsys*(restart+1) vectors)."""
+    t0 = time.perf_counter()
+    A = be.laplace3d(n, n, n)
+    N = A.nrows
+    print(f"bench: Laplacian {n}^3 ({N} rows) built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    shifts = np.linspace(0.05, 1.0, nsys)
+    m = A.domain_map
+    B = be.mvec(m, nsys)
+    B.random()
+    results = {}
+    for label, fp32 in (("fp64", False), ("fp32", True)):
+        X = be.mvec(m, nsys)
+        if fp32:  # cached FP32 copy of A: built outside the timed solve
+            b1, x1 = be.mvec(m, 1), be.mvec(m, 1)
+            be.minres_shifted(A, [0.0], b1, x1, 1.0, 1, fp32=True)
+            b1.delete()
+            x1.delete()
+        be.synchronize()
+        t0 = time.perf_counter()
+        it, rr, unconv = be.gmres_shifted(A, shifts, B, X, tol, max_iter, restart=restart, fp32=fp32)
+        be.synchronize()
+        dt = time.perf_counter() - t0
+        iters = int(it.max())
+        results[label] = {"seconds": round(dt, 3), "max_iters": iters, "median_iters": int(np.median(it)),
+                          "unconverged": int(unconv), "max_relres": float(rr.max()),
+                          "ms_per_step": round(1e3 * dt / max(1, iters), 4)}
+        print(f"bench: gmres {label}: {dt:.2f} s, max {iters} Arnoldi steps, {unconv} unconverged",
+              file=sys.stderr, flush=True)
+        X.delete()
+    out = {"metric": "blocked shifted GMRES time (16 shifts, tol 1e-8)", "value": results["fp64"]["seconds"],
+           "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64 and f32",
+           "data": "synthetic (3D Laplacian, KISS random right-hand sides)",
+           "config": {"workload": f"3D Laplacian {n}^3 ({N} rows), {nsys} shifts in [0.05,1.0], GMRES({restart}), "
+                                  f"tol {tol}, maxIter {max_iter}, zero initial guess"},
+           "fp64": results["fp64"], "fp32": results["fp32"]}
+    print(json.dumps(out), flush=True)
+    B.delete()
+    A.delete()
+
+
 def carp_bytes(nnz, n, nvec, s=8):
     """bytes of one forward + backward Kaczmarz sweep: every row is visited twice and reads its index
     (4), row pointer (8), entries (s+4 each) and, per column of X, reads and writes x at each entry"""
@@ -522,7 +565,7 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada", "minres", "serve", "carp"],
+    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada", "minres", "serve", "carp", "gmres"],
                     default="spmv",
                     help="spmv (default): the fused SpMV line; gram / cholqr / mvsd (fused_mvsd_mvTmv, "
                          "W = V*C and W'W, beta=0): the tall-skinny configuration (16,777,216 x k, U(-1,1)) "
@@ -560,6 +603,8 @@ def main():
         return serve_bench(be, args)
     if args.op == "carp":
         return carp_bench(be, args)
+    if args.op == "gmres":
+        return gmres_bench(be, args)
     if args.op != "spmv":
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
