@@ -20,7 +20,9 @@
 // Convergence as the reference (:600-611, :968-988): ||r|| / ||r0|| < tol or
 // ||r|| < 1000 eps (absolute); a system stops when its iterations reach
 // maxIter.  At the end of a cycle x_j += V_j y_j (y from the triangular
-// solve); a restart recomputes the true residual b - (A + sI) x.
+// solve); a restart recomputes the true residual b - (A + sI) x, and the
+// systems that finished leave: the next cycle runs on a block of the others
+// only (the basis shrinks with them).
 #include "solvers.hpp"
 #include <cmath>
 #include <limits>
@@ -36,163 +38,206 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
   PHX_REQUIRE((int64_t)(m + 1) * k <= 4096, PHIST_INVALID_INPUT, "GMRES basis (m+1)*k = %d columns too wide",
               (m + 1) * k);
   const double eps = std::numeric_limits<double>::epsilon();
-  // working precision copies of b and x
-  MvBox<TI> b(map, k), x(map, k), w(map, k), Vb(map, (m + 1) * k);
+  // working precision copies of b and x (all systems); the cycles run on
+  // compact blocks of the systems still iterating (act: their indices)
+  MvBox<TI> b(map, k), x(map, k);
   copy_any<T, TI>(B, b.get());
   copy_any<T, TI>(X, x.get());
-  std::vector<TI> s(shifts.begin(), shifts.end());
 
   GmresOut out;
   out.iters.assign(k, 0);
   out.relres.assign(k, 1.0);
-  std::vector<double> normR0(k, -1.0), normR(k, 0.0);
+  std::vector<double> normR0(k, -1.0);
   std::vector<int> status(k, 1);  // 1 iterating, 0 converged, 3 failed
-  // Hessenberg (column-major (m+1) x m per system), rotations, rhs
-  std::vector<double> H((size_t)k * (m + 1) * m), cs((size_t)k * m), sn((size_t)k * m), g((size_t)k * (m + 1));
-  auto h = [&](int j, int r, int c) -> double& { return H[((size_t)j * m + c) * (m + 1) + r]; };
 
-  auto residual = [&](std::vector<double>& nrm) {  // w = b - (A + sI) x, norms
-    op.apply_shifted(TI(-1), s.data(), x.get(), TI(0), w.get(), nullptr);
-    std::vector<TI> one(k, TI(1));
-    blas_axpby<TI>(one.data(), true, b.get(), TI(1), w.get());
-    std::vector<TI> d(k);
-    blas_dot<TI>(w.get(), w.get(), d.data());
-    for (int j = 0; j < k; j++) nrm[j] = std::sqrt((double)d[j]);
+  std::vector<int> act(k);
+  for (int j = 0; j < k; j++) act[j] = j;
+  int kk = k;
+  auto cols = [&](Mvec<TI>* src, const std::vector<int>& idx, Mvec<TI>* dst) {  // dst(:, a) = src(:, idx[a])
+    for (size_t a = 0; a < idx.size(); a++) {
+      View<TI> from(src, idx[a], idx[a]), to(dst, (int)a, (int)a);
+      blas_copy<TI>(from.get(), to.get());
+    }
+  };
+  std::unique_ptr<MvBox<TI>> bc(new MvBox<TI>(map, k)), xc(new MvBox<TI>(map, k)), wc(new MvBox<TI>(map, k));
+  blas_copy<TI>(b.get(), bc->get());
+  blas_copy<TI>(x.get(), xc->get());
+  std::vector<TI> s(shifts.begin(), shifts.end());
+
+  auto residual = [&](std::vector<double>& nrm) {  // w = b - (A + sI) x on the compact blocks, norms
+    op.apply_shifted(TI(-1), s.data(), xc->get(), TI(0), wc->get(), nullptr);
+    std::vector<TI> one(kk, TI(1));
+    blas_axpby<TI>(one.data(), true, bc->get(), TI(1), wc->get());
+    std::vector<TI> d(kk);
+    blas_dot<TI>(wc->get(), wc->get(), d.data());
+    nrm.resize(kk);
+    for (int a = 0; a < kk; a++) nrm[a] = std::sqrt((double)d[a]);
   };
 
-  std::vector<double> rn(k);
+  std::vector<double> rn;
   residual(rn);
   for (;;) {
-    // ---- (re)start: v_0 = r / ||r|| for the systems still iterating ----
-    std::vector<TI> sc(k);
-    for (int j = 0; j < k; j++) {
-      normR[j] = rn[j];
-      if (normR0[j] < 0.0) normR0[j] = rn[j];
+    // ---- convergence at the true residual, then the systems that go on ----
+    std::vector<int> keep;
+    for (int a = 0; a < kk; a++) {
+      const int j = act[a];
+      if (normR0[j] < 0.0) normR0[j] = rn[a];
+      const double rel = normR0[j] > 0.0 ? rn[a] / normR0[j] : 0.0;
+      out.relres[j] = rel;
       if (status[j] == 1) {
-        const double rel = normR0[j] > 0.0 ? rn[j] / normR0[j] : 0.0;
-        out.relres[j] = rel;
-        if (rn[j] < 1000 * eps || rel < tol[j]) status[j] = 0;
+        if (rn[a] < 1000 * eps || rel < tol[j]) status[j] = 0;
         else if (out.iters[j] >= maxIter) status[j] = 3;
       }
-      sc[j] = (status[j] == 1 && rn[j] > 0.0) ? TI(1.0 / rn[j]) : TI(0);
-      for (int r = 0; r <= m; r++) g[(size_t)j * (m + 1) + r] = 0.0;
-      g[(size_t)j * (m + 1)] = status[j] == 1 ? rn[j] : 0.0;
+      if (status[j] == 1) keep.push_back(a);
     }
-    bool any = false;
-    for (int j = 0; j < k; j++) any = any || status[j] == 1;
-    if (!any) break;
+    if ((int)keep.size() < kk) {
+      // finished systems leave with their solutions; the others move to narrower blocks
+      std::vector<int> done;
+      for (int a = 0, p = 0; a < kk; a++) {
+        if (p < (int)keep.size() && keep[p] == a) {
+          p++;
+          continue;
+        }
+        done.push_back(a);
+      }
+      for (int a : done) {
+        View<TI> from(xc->get(), a, a), to(x.get(), act[a], act[a]);
+        blas_copy<TI>(from.get(), to.get());
+      }
+      if (keep.empty()) break;
+      const int kn = (int)keep.size();
+      std::unique_ptr<MvBox<TI>> bn(new MvBox<TI>(map, kn)), xn(new MvBox<TI>(map, kn)), wn(new MvBox<TI>(map, kn));
+      cols(bc->get(), keep, bn->get());
+      cols(xc->get(), keep, xn->get());
+      cols(wc->get(), keep, wn->get());
+      std::vector<int> an(kn);
+      std::vector<TI> sn_(kn);
+      std::vector<double> rnn(kn);
+      for (int a = 0; a < kn; a++) {
+        an[a] = act[keep[a]];
+        sn_[a] = s[keep[a]];
+        rnn[a] = rn[keep[a]];
+      }
+      bc.swap(bn);
+      xc.swap(xn);
+      wc.swap(wn);
+      act.swap(an);
+      s.swap(sn_);
+      rn.swap(rnn);
+      kk = kn;
+    }
+
+    // ---- one GMRES(m) cycle on the kk systems: v_0 = r / ||r|| ----
+    MvBox<TI> w(map, kk), Vb(map, (m + 1) * kk);
+    std::vector<double> H((size_t)kk * (m + 1) * m), cs((size_t)kk * m), sn((size_t)kk * m), g((size_t)kk * (m + 1));
+    auto h = [&](int a, int r, int c) -> double& { return H[((size_t)a * m + c) * (m + 1) + r]; };
+    std::vector<TI> sc(kk);
+    for (int a = 0; a < kk; a++) {
+      sc[a] = rn[a] > 0.0 ? TI(1.0 / rn[a]) : TI(0);
+      g[(size_t)a * (m + 1)] = rn[a];
+    }
     {
-      View<TI> v0(Vb.get(), 0, k - 1);
-      blas_axpby<TI>(sc.data(), false, w.get(), TI(0), v0.get());
+      View<TI> v0(Vb.get(), 0, kk - 1);
+      blas_axpby<TI>(sc.data(), false, wc->get(), TI(0), v0.get());
     }
-    std::vector<int> active(k), steps(k, 0);
-    for (int j = 0; j < k; j++) active[j] = status[j] == 1;
+    std::vector<int> active(kk, 1), steps(kk, 0);
     int i = 0;
     for (; i < m; i++) {
       bool live = false;
-      for (int j = 0; j < k; j++) live = live || active[j];
+      for (int a = 0; a < kk; a++) live = live || active[a];
       if (!live) break;
-      View<TI> vi(Vb.get(), i * k, (i + 1) * k - 1);
-      View<TI> Vi(Vb.get(), 0, (i + 1) * k - 1);
+      View<TI> vi(Vb.get(), i * kk, (i + 1) * kk - 1);
+      View<TI> Vi(Vb.get(), 0, (i + 1) * kk - 1);
       op.apply_shifted(TI(1), s.data(), vi.get(), TI(0), w.get(), nullptr);
-      std::vector<double> hc((size_t)(i + 1) * k, 0.0);
+      std::vector<double> hc((size_t)(i + 1) * kk, 0.0);
       for (int pass = 0; pass < 2; pass++) {  // classical Gram-Schmidt, twice
-        SdBox<TI> G((i + 1) * k, k);
+        SdBox<TI> G((i + 1) * kk, kk);
         gemm_gram<TI>(TI(1), Vi.get(), w.get(), TI(0), G.get());
-        for (int j = 0; j < k; j++) {
+        for (int a = 0; a < kk; a++) {
           for (int l = 0; l <= i; l++) {
-            const TI c = active[j] ? G->h(l * k + j, j) : TI(0);
-            hc[(size_t)l * k + j] += (double)c;
+            const TI c = active[a] ? G->h(l * kk + a, a) : TI(0);
+            hc[(size_t)l * kk + a] += (double)c;
           }
         }
-        SdBox<TI> C((i + 1) * k, k);
-        for (int j = 0; j < k; j++)
-          for (int l = 0; l <= i; l++) C->h(l * k + j, j) = active[j] ? G->h(l * k + j, j) : TI(0);
+        SdBox<TI> C((i + 1) * kk, kk);
+        for (int a = 0; a < kk; a++)
+          for (int l = 0; l <= i; l++) C->h(l * kk + a, a) = active[a] ? G->h(l * kk + a, a) : TI(0);
         sd_to_device(C.get());
         gemm_times_sd<TI>(TI(-1), Vi.get(), C.get(), TI(1), w.get());
       }
-      std::vector<TI> d(k), inv(k);
+      std::vector<TI> d(kk), inv(kk);
       blas_dot<TI>(w.get(), w.get(), d.data());
-      for (int j = 0; j < k; j++) {
-        const double hn = std::sqrt((double)d[j]);
-        inv[j] = (active[j] && hn > 0.0 && std::isfinite(hn)) ? TI(1.0 / hn) : TI(0);
-        if (!active[j]) continue;
+      for (int a = 0; a < kk; a++) {
+        const int j = act[a];
+        const double hn = std::sqrt((double)d[a]);
+        inv[a] = (active[a] && hn > 0.0 && std::isfinite(hn)) ? TI(1.0 / hn) : TI(0);
+        if (!active[a]) continue;
         if (!std::isfinite(hn)) {  // broken down: keep the last solution of this cycle
-          active[j] = 0;
+          active[a] = 0;
           status[j] = 3;
           continue;
         }
         // Hessenberg column i: h(0:i, i) and h(i+1, i); earlier rotations, then a new one
-        for (int l = 0; l <= i; l++) h(j, l, i) = hc[(size_t)l * k + j];
-        h(j, i + 1, i) = hn;
+        for (int l = 0; l <= i; l++) h(a, l, i) = hc[(size_t)l * kk + a];
+        h(a, i + 1, i) = hn;
         for (int l = 0; l < i; l++) {
-          const double c = cs[(size_t)j * m + l], sv = sn[(size_t)j * m + l];
-          const double a = h(j, l, i), bb = h(j, l + 1, i);
-          h(j, l, i) = c * a + sv * bb;
-          h(j, l + 1, i) = -sv * a + c * bb;
+          const double c = cs[(size_t)a * m + l], sv = sn[(size_t)a * m + l];
+          const double p0 = h(a, l, i), p1 = h(a, l + 1, i);
+          h(a, l, i) = c * p0 + sv * p1;
+          h(a, l + 1, i) = -sv * p0 + c * p1;
         }
-        const double a = h(j, i, i), bb = h(j, i + 1, i);
-        const double r = std::hypot(a, bb);
-        const double c = r > 0.0 ? a / r : 1.0, sv = r > 0.0 ? bb / r : 0.0;
-        cs[(size_t)j * m + i] = c;
-        sn[(size_t)j * m + i] = sv;
-        h(j, i, i) = r;
-        h(j, i + 1, i) = 0.0;
-        double* gj = &g[(size_t)j * (m + 1)];
-        gj[i + 1] = -sv * gj[i];
-        gj[i] = c * gj[i];
-        steps[j] = i + 1;
+        const double p0 = h(a, i, i), p1 = h(a, i + 1, i);
+        const double r = std::hypot(p0, p1);
+        const double c = r > 0.0 ? p0 / r : 1.0, sv = r > 0.0 ? p1 / r : 0.0;
+        cs[(size_t)a * m + i] = c;
+        sn[(size_t)a * m + i] = sv;
+        h(a, i, i) = r;
+        h(a, i + 1, i) = 0.0;
+        double* ga = &g[(size_t)a * (m + 1)];
+        ga[i + 1] = -sv * ga[i];
+        ga[i] = c * ga[i];
+        steps[a] = i + 1;
         out.iters[j]++;
-        normR[j] = std::fabs(gj[i + 1]);
-        const double rel = normR[j] / normR0[j];
+        const double nr = std::fabs(ga[i + 1]);
+        const double rel = nr / normR0[j];
         out.relres[j] = rel;
-        if (normR[j] < 1000 * eps || rel < tol[j] || hn == 0.0) {
-          active[j] = 0;
+        if (nr < 1000 * eps || rel < tol[j] || hn == 0.0) {
+          active[a] = 0;
           status[j] = 0;
         } else if (out.iters[j] >= maxIter) {
-          active[j] = 0;
+          active[a] = 0;
           status[j] = 3;
         }
       }
-      View<TI> vn(Vb.get(), (i + 1) * k, (i + 2) * k - 1);
+      View<TI> vn(Vb.get(), (i + 1) * kk, (i + 2) * kk - 1);
       blas_axpby<TI>(inv.data(), false, w.get(), TI(0), vn.get());
-      for (int j = 0; j < k; j++)  // a non-finite column must not reach the other systems through 0 * NaN
-        if (!std::isfinite((double)d[j])) {
-          View<TI> col(vn.get(), j, j);
+      for (int a = 0; a < kk; a++)  // a non-finite column must not reach the other systems through 0 * NaN
+        if (!std::isfinite((double)d[a])) {
+          View<TI> col(vn.get(), a, a);
           blas_put_value<TI>(col.get(), TI(0));
         }
     }
     // ---- x += V y per system (y from the triangular systems R y = g) ----
     int L = 0;
-    for (int j = 0; j < k; j++) L = std::max(L, steps[j]);
+    for (int a = 0; a < kk; a++) L = std::max(L, steps[a]);
     if (L > 0) {
-      SdBox<TI> Y(L * k, k);
-      for (int j = 0; j < k; j++) {
-        const int n = steps[j];
+      SdBox<TI> Y(L * kk, kk);
+      for (int a = 0; a < kk; a++) {
+        const int n = steps[a];
         std::vector<double> y(n);
         for (int r = n - 1; r >= 0; r--) {
-          double acc = g[(size_t)j * (m + 1) + r];
-          for (int c = r + 1; c < n; c++) acc -= h(j, r, c) * y[c];
-          y[r] = h(j, r, r) != 0.0 ? acc / h(j, r, r) : 0.0;
+          double acc = g[(size_t)a * (m + 1) + r];
+          for (int c = r + 1; c < n; c++) acc -= h(a, r, c) * y[c];
+          y[r] = h(a, r, r) != 0.0 ? acc / h(a, r, r) : 0.0;
         }
-        for (int r = 0; r < n; r++) Y->h(r * k + j, j) = (TI)y[r];
+        for (int r = 0; r < n; r++) Y->h(r * kk + a, a) = (TI)y[r];
       }
       sd_to_device(Y.get());
-      View<TI> VL(Vb.get(), 0, L * k - 1);
-      gemm_times_sd<TI>(TI(1), VL.get(), Y.get(), TI(1), x.get());
+      View<TI> VL(Vb.get(), 0, L * kk - 1);
+      gemm_times_sd<TI>(TI(1), VL.get(), Y.get(), TI(1), xc->get());
     }
-    // ---- restart with the true residual (or finish) ----
-    bool again = false;
-    for (int j = 0; j < k; j++) again = again || status[j] == 1;
+    // ---- the true residual: restart, or the final residuals of the systems that stopped ----
     residual(rn);
-    if (!again) {
-      for (int j = 0; j < k; j++)
-        if (normR0[j] > 0.0) out.relres[j] = rn[j] / normR0[j];
-      break;
-    }
-    // a system the implicit residual declared converged keeps that status
-    for (int j = 0; j < k; j++)
-      if (status[j] != 1) rn[j] = 0.0;
   }
   copy_any<TI, T>(x.get(), X);
   out.unconverged = 0;
