@@ -112,6 +112,9 @@ struct Runtime {
   // 2.7M rows for 1..4 output columns from 8..44 inputs, FP64 and FP32)
   int mix_mfma_min_m = 1;
   bool minres_device = true;  // MINRES scalars on the device for this library's matrices
+  // blocked GMRES: Gram-Schmidt with the per-system sweeps (gmres_cgs_dev)
+  // where the block allows, else with Grams and row mixes of the whole basis
+  bool gmres_cgs = true;
   // device MINRES: iterations enqueued per iterate() call (at most; fewer when
   // no active system can take that many more), and whether runs of
   // minres_graph_len iterations are replayed from captured HIP graphs instead
@@ -300,6 +303,10 @@ struct SpmvRequest {
   const double* xscale = nullptr;
   int64_t xscale_stride = 1;
 };
+
+// limits of the blocked GMRES Gram-Schmidt sweeps (systems per block, basis
+// vectors per system held in registers)
+constexpr int kGmresCgsMaxKk = 64, kGmresCgsMaxNl = 32;
 
 // per-column state of the device-resident MINRES recurrences (Paige & Saunders)
 struct MinresState {

@@ -8,13 +8,15 @@
 // columns of block V_l, stored side by side in one block of (m+1)*k columns
 // so one pass reads every basis vector of a row.  Per step:
 //   w = (A + diag(s)) v_i                               one shifted SpMV
-//   h(0:i) = V(0:i)' w, w -= V(0:i) h  (twice, CGS2)     Gram + row mix on the
-//                                                        whole basis; only the
-//                                                        blocks' diagonals are
-//                                                        used (the other
-//                                                        entries ride along in
-//                                                        the memory-bound pass)
+//   h(0:i) = V(0:i)' w, w -= V(0:i) h  (twice, CGS2)     three sweeps over the
+//                                                        basis (gmres_cgs_dev:
+//                                                        dots; update + dots;
+//                                                        update + norms), each
+//                                                        system meeting only
+//                                                        its own columns
 //   h(i+1) = ||w||, v_{i+1} = w / h(i+1)
+// (for more than 64 systems or m > 32: a Gram and a row mix of the whole
+// basis per pass, of which only the blocks' diagonals are used).
 //   Givens rotations of every system's Hessenberg column on the host; the
 //   implicit residual |g_{i+1}| decides convergence.
 // Convergence as the reference (:600-611, :968-988): ||r|| / ||r0|| < tol or
@@ -141,6 +143,31 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
       blas_axpby<TI>(sc.data(), false, wc->get(), TI(0), v0.get());
     }
     std::vector<int> active(kk, 1), steps(kk, 0);
+    // per-system sweeps (each system meets only its own basis): device work for the partial sums, the
+    // coefficients of both passes and the squared norms (D1 | D2 | N, read back together), the mask of
+    // the systems still iterating and the solution coefficients
+    const bool cgs = rt().gmres_cgs && kk <= kGmresCgsMaxKk && m <= kGmresCgsMaxNl;
+    const int64_t nrows = w->nrows, mk = (int64_t)m * kk;
+    const int64_t work_elems = cgs ? (int64_t)gmres_cgs_grid(nrows, kk) * mk + mk : 0;
+    const size_t dev_elems = cgs ? (size_t)(work_elems + 2 * mk + kk + kk + mk) : 0;
+    TI* dbuf = cgs ? static_cast<TI*>(dev_alloc(sizeof(TI) * dev_elems)) : nullptr;
+    struct Release {
+      void* p;
+      size_t n;
+      ~Release() { dev_release(p, n); }
+    } release{dbuf, sizeof(TI) * dev_elems};
+    TI* d_outs = dbuf + work_elems;
+    TI* d_act = d_outs + 2 * mk + kk;
+    TI* d_y = d_act + kk;
+    auto upload = [&](TI* dst, const TI* src, int64_t n) {
+      const size_t bytes = sizeof(TI) * n;
+      if (!stage_upload_2d(dst, bytes, src, bytes, bytes, 1)) {
+        PHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, rt().stream));
+        PHX_HIP(hipStreamSynchronize(rt().stream));
+      }
+    };
+    std::vector<TI> hact(kk, TI(1));
+    if (cgs) upload(d_act, hact.data(), kk);
     int i = 0;
     for (; i < m; i++) {
       bool live = false;
@@ -150,7 +177,34 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
       View<TI> Vi(Vb.get(), 0, (i + 1) * kk - 1);
       op.apply_shifted(TI(1), s.data(), vi.get(), TI(0), w.get(), nullptr);
       std::vector<double> hc((size_t)(i + 1) * kk, 0.0);
-      for (int pass = 0; pass < 2; pass++) {  // classical Gram-Schmidt, twice
+      std::vector<TI> d(kk), inv(kk);
+      if (cgs) {  // classical Gram-Schmidt twice in three sweeps, the norms in the last
+        bool changed = false;
+        for (int a = 0; a < kk; a++) {
+          const TI f = active[a] ? TI(1) : TI(0);
+          changed = changed || f != hact[a];
+          hact[a] = f;
+        }
+        if (changed) upload(d_act, hact.data(), kk);
+        const int nl = i + 1, nc = nl * kk;
+        TI* D1 = d_outs;
+        TI* D2 = d_outs + nc;
+        TI* N = d_outs + 2 * nc;
+        const TI* v = Vb->dev();
+        const int64_t ldv = Vb->lda;
+        gmres_cgs_dev<TI>(0, nrows, v, ldv, w->dev(), w->lda, kk, nl, nullptr, nullptr, dbuf, work_elems, D1);
+        gmres_cgs_dev<TI>(1, nrows, v, ldv, w->dev(), w->lda, kk, nl, D1, d_act, dbuf, work_elems, D2);
+        gmres_cgs_dev<TI>(2, nrows, v, ldv, w->dev(), w->lda, kk, nl, D2, d_act, dbuf, work_elems, N);
+        TI* hb = static_cast<TI*>(host_work(sizeof(TI) * (2 * nc + kk)));
+        PHX_HIP(hipMemcpyAsync(hb, d_outs, sizeof(TI) * (2 * nc + kk), hipMemcpyDeviceToHost, rt().stream));
+        PHX_HIP(hipStreamSynchronize(rt().stream));
+        for (int a = 0; a < kk; a++) {
+          if (active[a])
+            for (int l = 0; l <= i; l++) hc[(size_t)l * kk + a] = (double)hb[l * kk + a] + (double)hb[nc + l * kk + a];
+          d[a] = hb[2 * nc + a];
+        }
+      }
+      for (int pass = 0; pass < 2 && !cgs; pass++) {  // classical Gram-Schmidt, twice
         SdBox<TI> G((i + 1) * kk, kk);
         gemm_gram<TI>(TI(1), Vi.get(), w.get(), TI(0), G.get());
         for (int a = 0; a < kk; a++) {
@@ -165,8 +219,7 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
         sd_to_device(C.get());
         gemm_times_sd<TI>(TI(-1), Vi.get(), C.get(), TI(1), w.get());
       }
-      std::vector<TI> d(kk), inv(kk);
-      blas_dot<TI>(w.get(), w.get(), d.data());
+      if (!cgs) blas_dot<TI>(w.get(), w.get(), d.data());
       for (int a = 0; a < kk; a++) {
         const int j = act[a];
         const double hn = std::sqrt((double)d[a]);
@@ -220,16 +273,31 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
     // ---- x += V y per system (y from the triangular systems R y = g) ----
     int L = 0;
     for (int a = 0; a < kk; a++) L = std::max(L, steps[a]);
-    if (L > 0) {
+    auto ysolve = [&](int a) {  // back substitution with the rotated Hessenberg matrix of system a
+      const int n = steps[a];
+      std::vector<double> y(n);
+      for (int r = n - 1; r >= 0; r--) {
+        double acc = g[(size_t)a * (m + 1) + r];
+        for (int c = r + 1; c < n; c++) acc -= h(a, r, c) * y[c];
+        y[r] = h(a, r, r) != 0.0 ? acc / h(a, r, r) : 0.0;
+      }
+      return y;
+    };
+    if (L > 0 && cgs) {  // x -= V c with c(l, a) = -y_a(l): one sweep over the basis
+      std::vector<TI> yc((size_t)L * kk, TI(0));
+      for (int a = 0; a < kk; a++) {
+        const int n = steps[a];
+        const std::vector<double> y = ysolve(a);
+        for (int r = 0; r < n; r++) yc[(size_t)r * kk + a] = (TI)(-y[r]);
+      }
+      upload(d_y, yc.data(), (int64_t)L * kk);
+      gmres_cgs_dev<TI>(3, nrows, Vb->dev(), Vb->lda, xc->get()->dev(), xc->get()->lda, kk, L, d_y, nullptr, nullptr,
+                        0, nullptr);
+    } else if (L > 0) {
       SdBox<TI> Y(L * kk, kk);
       for (int a = 0; a < kk; a++) {
         const int n = steps[a];
-        std::vector<double> y(n);
-        for (int r = n - 1; r >= 0; r--) {
-          double acc = g[(size_t)a * (m + 1) + r];
-          for (int c = r + 1; c < n; c++) acc -= h(a, r, c) * y[c];
-          y[r] = h(a, r, r) != 0.0 ? acc / h(a, r, r) : 0.0;
-        }
+        const std::vector<double> y = ysolve(a);
         for (int r = 0; r < n; r++) Y->h(r * kk + a, a) = (TI)y[r];
       }
       sd_to_device(Y.get());

@@ -494,6 +494,185 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   launch(true, p, {v, vold, w, d1, d2, x});
 }
 
+// ---------------------------------------------------------------------------
+// Classical Gram-Schmidt of the blocked GMRES: kk systems, each with its own
+// basis, laid side by side (column l*kk + a of V is basis vector l of system
+// a).  System a only ever meets its own columns, so a sweep reads every basis
+// entry once and does one multiply-add with it (no Gram of the whole block):
+//   mode 0: D(l,a) = <V(:,l*kk+a), w(:,a)>
+//   mode 1: w(:,a) -= sum_l V(:,l*kk+a) C(l,a), then D(l,a) of the new w
+//   mode 2: as mode 1, then N(a) = ||new w(:,a)||^2
+//   mode 3: as mode 1 without outputs (x += V y at the end of a cycle)
+// with C(l,a) = coef[l*kk+a] where act[a] != 0, else 0 (act == nullptr: all).
+// Thread (rl, a) of a workgroup owns system a of row t*RP + rl (RP = 256/KP,
+// KP the next power of two >= kk) and keeps the row's nl basis entries in
+// registers (buffer loads with the vector's offset as scalar operand); the outputs are reduced per workgroup (lanes of a wave that
+// share a, then the four waves in order) and over workgroups by
+// k_reduce_partials, so every sum has a fixed order.
+template <typename T>
+struct CgsArgs {
+  int64_t nrows, ntiles;
+  const T* v;
+  int64_t ldv;
+  T* w;
+  int64_t ldw;
+  int kk, kp, nl;
+  const T* coef;
+  const T* act;
+  T* partials;
+};
+
+template <typename T, int LMAX, int MODE>
+__global__ void __launch_bounds__(kBlock) k_gmres_cgs(const CgsArgs<T> p) {
+  extern __shared__ unsigned char cgs_lds[];
+  T* sh = reinterpret_cast<T*>(cgs_lds);
+  const int kp = p.kp, rp = kBlock / kp;
+  const int a = threadIdx.x % kp, rl = threadIdx.x / kp;
+  const bool live = a < p.kk;
+  // coefficients in LDS after the reduction area (registers hold the row's basis entries and the sums)
+  // (zero for l >= nl, so the loads of those slots, which repeat vector nl-1, add nothing)
+  T* cf = sh + (MODE == 3 ? 0 : (MODE == 2 ? p.kk : p.nl * p.kk));
+  if (MODE > 0) {
+    for (int e = threadIdx.x; e < LMAX * p.kk; e += kBlock) {
+      const int c = e % p.kk;
+      cf[e] = (e < p.nl * p.kk && (!p.act || p.act[c] != T(0))) ? p.coef[e] : T(0);
+    }
+    __syncthreads();
+  }
+  T acc[LMAX];
+  T nacc = T(0);
+#pragma unroll
+  for (int l = 0; l < LMAX; l++) acc[l] = T(0);
+  const TileWalk tw = xcd_tiles(p.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    const int64_t r = t * rp + rl;
+    if (!live || r >= p.nrows) continue;
+    // buffer loads: the tile's rows as the resource (uniform base), the lane's offset in one register
+    // and the vector's offset l*kk (clamped to nl-1: every slot loads, no branch) as the scalar offset
+    const int64_t rows = p.nrows - t * rp < rp ? p.nrows - t * rp : rp;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(p.v + t * rp * p.ldv), (short)0, (int)(rows * p.ldv * (int64_t)sizeof(T)), 0x00020000);
+    const int vo = (int)((rl * p.ldv + a) * sizeof(T));
+    T vv[LMAX];
+#pragma unroll
+    for (int l = 0; l < LMAX; l++) {
+      const int so = (l < p.nl ? l : p.nl - 1) * p.kk * (int)sizeof(T);
+      if constexpr (sizeof(T) == 8)
+        vv[l] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0));
+      else
+        vv[l] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0));
+    }
+    T* wr = p.w + r * p.ldw + a;
+    T s = *wr;
+    // keep every load of the row in flight together (the scheduler would otherwise pair each load with
+    // its use to save registers, one memory latency per vector)
+    __builtin_amdgcn_sched_barrier(0);
+    if (MODE > 0) {
+#pragma unroll
+      for (int l = 0; l < LMAX; l++) s -= vv[l] * cf[l * p.kk + a];
+      *wr = s;
+    }
+    if (MODE == 0 || MODE == 1) {
+#pragma unroll
+      for (int l = 0; l < LMAX; l++) acc[l] += vv[l] * s;
+    } else if (MODE == 2) {
+      nacc += s * s;
+    }
+  }
+  if (MODE == 3) return;
+  // workgroup sums: lanes of a wave with the same a, then the waves in order
+  const int nout = MODE == 2 ? p.kk : p.nl * p.kk;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int nouts = MODE == 2 ? 1 : LMAX;
+  if (MODE == 2) acc[0] = nacc;
+  // (acc is indexed by compile-time l only, so it stays in registers)
+  for (int off = kp; off < 64; off <<= 1) {
+#pragma unroll
+    for (int l = 0; l < nouts; l++) acc[l] += __shfl_xor(acc[l], off, 64);
+  }
+#pragma unroll 1
+  for (int w = 0; w < kBlock / 64; w++) {
+    if (wv == w && lane < kp && live) {
+#pragma unroll
+      for (int l = 0; l < nouts; l++) {
+        if (MODE == 2 || l < p.nl) {
+          const int o = MODE == 2 ? a : l * p.kk + a;
+          sh[o] = w == 0 ? acc[l] : sh[o] + acc[l];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int o = threadIdx.x; o < nout; o += kBlock) p.partials[(int64_t)blockIdx.x * nout + o] = sh[o];
+}
+
+template <typename T, int LMAX, int MODE>
+static void cgs_launch(CgsArgs<T> p, int grid, size_t lds) {
+  hipLaunchKernelGGL((k_gmres_cgs<T, LMAX, MODE>), dim3(grid), dim3(kBlock), lds, rt().stream, p);
+  PHX_HIP(hipGetLastError());
+}
+
+template <typename T, int MODE>
+static void cgs_lmax(CgsArgs<T> p, int grid, size_t lds) {
+  switch ((p.nl + 3) / 4) {  // register slots: nl rounded up to a multiple of 4
+    case 1: cgs_launch<T, 4, MODE>(p, grid, lds); break;
+    case 2: cgs_launch<T, 8, MODE>(p, grid, lds); break;
+    case 3: cgs_launch<T, 12, MODE>(p, grid, lds); break;
+    case 4: cgs_launch<T, 16, MODE>(p, grid, lds); break;
+    case 5: cgs_launch<T, 20, MODE>(p, grid, lds); break;
+    case 6: cgs_launch<T, 24, MODE>(p, grid, lds); break;
+    case 7: cgs_launch<T, 28, MODE>(p, grid, lds); break;
+    default: cgs_launch<T, 32, MODE>(p, grid, lds); break;
+  }
+}
+
+// Workgroups of a gmres_cgs_dev sweep (the partial sums need grid * nout
+// elements of work).
+int gmres_cgs_grid(int64_t nrows, int kk) {
+  int kp = 1;
+  while (kp < kk) kp <<= 1;
+  const int64_t ntiles = (nrows + kBlock / kp - 1) / (kBlock / kp);
+  return persistent_blocks(k_gmres_cgs<double, 32, 1>, 2 * kGmresCgsMaxNl * 64 * sizeof(double), ntiles, 4);
+}
+
+template <typename T>
+void gmres_cgs_dev(int mode, int64_t nrows, const T* v, int64_t ldv, T* w, int64_t ldw, int kk, int nl,
+                   const T* coef, const T* act, T* work, int64_t work_elems, T* out) {
+  PHX_REQUIRE(kk >= 1 && kk <= kGmresCgsMaxKk && nl >= 1 && nl <= kGmresCgsMaxNl && mode >= 0 && mode <= 3,
+              PHIST_INVALID_INPUT, "Gram-Schmidt sweep over %d systems x %d vectors (mode %d)", kk, nl, mode);
+  PHX_REQUIRE((int64_t)kBlock * ldv * (int64_t)sizeof(T) < (int64_t(1) << 31), PHIST_INVALID_INPUT,
+              "basis row stride %ld too large", (long)ldv);  // a tile's rows fit one buffer resource
+  CgsArgs<T> p;
+  p.nrows = nrows;
+  p.v = v;
+  p.ldv = ldv;
+  p.w = w;
+  p.ldw = ldw;
+  p.kk = kk;
+  p.kp = 1;
+  while (p.kp < kk) p.kp <<= 1;
+  p.nl = nl;
+  p.coef = coef;
+  p.act = act;
+  p.partials = work;
+  p.ntiles = (nrows + kBlock / p.kp - 1) / (kBlock / p.kp);
+  const int nout = mode == 2 ? kk : nl * kk;
+  const int grid = gmres_cgs_grid(nrows, kk);
+  PHX_REQUIRE(mode == 3 || (int64_t)grid * nout + nout <= work_elems, PHIST_FUNCTIONAL_ERROR,
+              "Gram-Schmidt work of %ld elements is too small", (long)work_elems);
+  const int lmax = (nl + 3) / 4 * 4;
+  const size_t lds = sizeof(T) * ((mode == 3 ? 0 : nout) + (mode > 0 ? (size_t)lmax * kk : 0));
+  switch (mode) {
+    case 0: cgs_lmax<T, 0>(p, grid, lds); break;
+    case 1: cgs_lmax<T, 1>(p, grid, lds); break;
+    case 2: cgs_lmax<T, 2>(p, grid, lds); break;
+    default: cgs_lmax<T, 3>(p, grid, lds); return;
+  }
+  hipLaunchKernelGGL(k_reduce_partials<T>, dim3((nout + 15) / 16), dim3(kBlock), 0, rt().stream, work, grid, nout,
+                     out);
+  PHX_HIP(hipGetLastError());
+}
+
 // Y = X converted to TD (same shape; any row strides)
 template <typename TS, typename TD>
 void convert_block(const TS* x, int64_t ldx, TD* y, int64_t ldy, int64_t nrows, int ncols) {
@@ -521,6 +700,10 @@ template void krylov_minres_step_dev<double>(int64_t, int, int64_t, const double
 template void krylov_minres_step_dev<float>(int64_t, int, int64_t, const float*, float*, float*, const float*, float*,
                                             float*, const float*, MinresState*, float*, const float*, int64_t, int,
                                             const float*, int, int);
+template void gmres_cgs_dev<double>(int, int64_t, const double*, int64_t, double*, int64_t, int, int,
+                                    const double*, const double*, double*, int64_t, double*);
+template void gmres_cgs_dev<float>(int, int64_t, const float*, int64_t, float*, int64_t, int, int, const float*,
+                                   const float*, float*, int64_t, float*);
 template void convert_block<double, float>(const double*, int64_t, float*, int64_t, int64_t, int);
 template void convert_block<float, double>(const float*, int64_t, double*, int64_t, int64_t, int);
 

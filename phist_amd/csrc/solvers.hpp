@@ -21,6 +21,19 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
 template <typename TS, typename TD>
 void convert_block(const TS* x, int64_t ldx, TD* y, int64_t ldy, int64_t nrows, int ncols);
 
+// Classical Gram-Schmidt sweeps of the blocked GMRES over kk bases laid side
+// by side (column l*kk + a of v: vector l of system a), one read of the basis
+// each; mode 0: out(l,a) = <v(:,l*kk+a), w(:,a)>; 1: w(:,a) -= sum_l
+// v(:,l*kk+a) c(l,a), then out as in mode 0 for the new w; 2: the update,
+// then out(a) = ||w(:,a)||^2; 3: the update only.  c(l,a) = coef[l*kk+a]
+// (device) where act[a] != 0 (act == nullptr: every system).  work holds
+// gmres_cgs_grid(nrows, kk) * nout partial sums plus nout; out (device) gets
+// the nout = nl*kk (mode 2: kk) sums.
+int gmres_cgs_grid(int64_t nrows, int kk);
+template <typename T>
+void gmres_cgs_dev(int mode, int64_t nrows, const T* v, int64_t ldv, T* w, int64_t ldw, int kk, int nl,
+                   const T* coef, const T* act, T* work, int64_t work_elems, T* out);
+
 // owned block vector, freed at scope exit
 template <typename T>
 struct MvBox {

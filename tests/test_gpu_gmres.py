@@ -21,13 +21,15 @@ def true_relres(A, shifts, B, X, X0=None):
     return np.linalg.norm(r, axis=0) / np.linalg.norm(r0, axis=0)
 
 
-@pytest.mark.parametrize("restart", [30, 8])
-def test_gmres_nonsymmetric_banded(be64, restart):
+# restart <= 32 with <= 64 systems: per-system Gram-Schmidt sweeps (4, 64 and 1 lanes per row);
+# restart 40: Grams and row mixes of the whole basis
+@pytest.mark.parametrize("restart,nsys", [(30, 4), (8, 4), (40, 4), (12, 40), (20, 1)])
+def test_gmres_nonsymmetric_banded(be64, restart, nsys):
     A = be64.banded_random(20000, 16, 300, 20.0, 5)  # nonsymmetric, diagonally dominant
     n = A.nrows
-    shifts = np.array([0.0, 1.5, -3.0, 6.0])
-    Bh = np.random.default_rng(1).uniform(-1, 1, (n, 4))
-    B, X = be64.mvec(A.domain_map, 4, Bh), be64.mvec(A.domain_map, 4)
+    shifts = np.array([0.0, 1.5, -3.0, 6.0]) if nsys == 4 else np.linspace(-3.0, 6.0, nsys)
+    Bh = np.random.default_rng(1).uniform(-1, 1, (n, nsys))
+    B, X = be64.mvec(A.domain_map, nsys, Bh), be64.mvec(A.domain_map, nsys)
     it, rr, unconv = be64.gmres_shifted(A, shifts, B, X, 1e-10, 500, restart=restart)
     assert unconv == 0, (it, rr)
     got = true_relres(A, shifts, Bh, X.get())
