@@ -285,8 +285,8 @@ __global__ void __launch_bounds__(kBlock) k_minres_rot(const T* __restrict__ dot
       if (q.relres <= q.tol || bn == 0.0 || q.iters >= q.max_iter) q.active = 0;
     }
   }
-  if (phase >= 0) q.sc[(phase + 1) % 3] = scn;
   st[c] = q;
+  if (phase >= 0) st[c].sc[(phase + 1) % 3] = scn;  // (a run-time index into the local copy would put it in scratch)
   coef[c] = dl;
   coef[k + c] = ep;
   coef[2 * k + c] = ig;
@@ -309,8 +309,8 @@ __global__ void k_minres_init(MinresState* __restrict__ st, const T* __restrict_
   q.phibar = nn;
   q.relres = nn > 0.0 ? 1.0 : 0.0;
   q.active = (nn > 0.0 && q.max_iter > 0) ? 1 : 0;
-  q.sc[ph] = nn > 0.0 ? 1.0 / nn : 0.0;
   st[j0 + c] = q;
+  st[j0 + c].sc[ph] = nn > 0.0 ? 1.0 / nn : 0.0;
 }
 
 template <typename T>
