@@ -302,6 +302,10 @@ struct SpmvRequest {
   // factor on the device, converted to the scalar type)
   const double* xscale = nullptr;
   int64_t xscale_stride = 1;
+  // when >= 0, columns from ncols_active on are neither read nor written
+  // (their dots are 0); the launch is the one for all columns, so the
+  // other columns' results do not depend on it
+  int ncols_active = -1;
 };
 
 // limits of the blocked GMRES Gram-Schmidt sweeps (systems per block, basis

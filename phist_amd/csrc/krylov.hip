@@ -58,6 +58,8 @@ struct KrylovArgs {
   // states and the fused SpMV dots (dots[ncols + c] = <v, Av>) instead of cdev
   const MinresState* st = nullptr;
   const T* dots = nullptr;
+  // lanes of columns >= nact (when >= 0) skip their rows (same launch and sums for the others)
+  int nact = -1;
   T c0[kKrylovMaxCols], c1[kKrylovMaxCols], c2[kKrylovMaxCols], c3[kKrylovMaxCols], c4[kKrylovMaxCols];
 };
 
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(const KrylovArgs<T> p) {
   constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = lane % LPR, rw = lane / LPR, c0 = q * VPL;
-  const bool colok = c0 < p.ncols;
+  const bool colok = c0 < (p.nact >= 0 ? p.nact : p.ncols);
   // projection coefficients C(j, c0+e) in LDS, row-interleaved: sc[j*VPL*LPR + c]
   extern __shared__ char lanczos_lds[];
   T* sc = reinterpret_cast<T*>(lanczos_lds);
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) k_minres(const KrylovArgs<T> p) {
   constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = lane % LPR, rw = lane / LPR, c0 = q * VPL;
-  if (c0 >= p.ncols) return;
+  if (c0 >= (p.nact >= 0 ? p.nact : p.ncols)) return;
   T dl[VPL], ep[VPL], ig[VPL], ta[VPL], ib[VPL], sv[VPL];
   const bool scaled = p.scv != nullptr;
 #pragma unroll
@@ -453,7 +455,7 @@ void krylov_minres_update(int64_t nrows, int ncols, int64_t ld, const T* v, T* v
 template <typename T>
 void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
                             const T* dots, MinresState* st, T* work, const T* Q, int64_t ldq, int nq, const T* C,
-                            int ldc, int phase) {
+                            int ldc, int phase, int nact) {
   PHX_REQUIRE(k >= 1 && k <= kKrylovMaxCols, PHIST_INVALID_INPUT, "Krylov block of %d columns", k);
   if (nrows == 0) return;  // every column starts frozen (zero right-hand side)
   hipStream_t s = rt().stream;
@@ -472,6 +474,7 @@ void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vol
   p.cdev = nullptr;
   p.st = st;  // a and b from the column states and the SpMV dots (no separate launch)
   p.dots = dots;
+  p.nact = nact >= 0 ? std::min(nact, k) : -1;
   p.q = Q;
   p.ldq = ldq;
   p.nq = Q ? nq : 0;
@@ -696,10 +699,10 @@ template void krylov_minres_update<float>(int64_t, int, int64_t, const float*, f
                                           const float*);
 template void krylov_minres_step_dev<double>(int64_t, int, int64_t, const double*, double*, double*, const double*,
                                              double*, double*, const double*, MinresState*, double*, const double*,
-                                             int64_t, int, const double*, int, int);
+                                             int64_t, int, const double*, int, int, int);
 template void krylov_minres_step_dev<float>(int64_t, int, int64_t, const float*, float*, float*, const float*, float*,
                                             float*, const float*, MinresState*, float*, const float*, int64_t, int,
-                                            const float*, int, int);
+                                            const float*, int, int, int);
 template void gmres_cgs_dev<double>(int, int64_t, const double*, int64_t, double*, int64_t, int, int,
                                     const double*, const double*, double*, int64_t, double*);
 template void gmres_cgs_dev<float>(int, int64_t, const float*, int64_t, float*, int64_t, int, int, const float*,

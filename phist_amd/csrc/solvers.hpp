@@ -187,7 +187,7 @@ struct Operator {
   // sparse matrices of this library only
   // (xscale: optional per-column factors of X on the device, stride xstride)
   void apply_shifted_dev(const T* s_dev, const Mvec<T>* X, Mvec<T>* Y, T* dots_dev, const double* xscale = nullptr,
-                         int64_t xstride = 1) const {
+                         int64_t xstride = 1, int ncols_active = -1) const {
     PHX_REQUIRE(A != nullptr, PHIST_NOT_IMPLEMENTED, "device dots need a sparse matrix of this library");
     SpmvRequest req;
     req.has_shift = true;
@@ -196,6 +196,7 @@ struct Operator {
     req.dots_dev = dots_dev;
     req.xscale = xscale;
     req.xscale_stride = xstride;
+    req.ncols_active = ncols_active;
     spmv_launch<T>(A, T(1), nullptr, X, T(0), Y, req, nullptr, nullptr);
   }
 };
@@ -235,7 +236,7 @@ void copy_any(const Mvec<TS>* X, Mvec<TD>* Y) {
 template <typename T>
 void krylov_minres_step_dev(int64_t nrows, int k, int64_t ld, const T* v, T* vold, T* w, const T* d1, T* d2, T* x,
                             const T* dots, MinresState* st, T* work, const T* Q = nullptr, int64_t ldq = 0,
-                            int nq = 0, const T* C = nullptr, int ldc = 0, int phase = -1);
+                            int nq = 0, const T* C = nullptr, int ldc = 0, int phase = -1, int nact = -1);
 
 // device column states of freshly loaded systems (see krylov.hip)
 template <typename T>
@@ -479,11 +480,22 @@ class MinresBatch {
   // the kernels of one iteration on the library stream (recorded instead of
   // run while a graph is captured); the roles of the rotating blocks advance
   void enqueue_iteration() {
+    // Without a projection (and outside graph capture) the vector passes skip the columns after the
+    // last one the host view shows active (the same launches, so the other columns' arithmetic does not
+    // change): the host view is a superset of the device's active set (a load shows active at once, a
+    // deactivation after the next copy), and an inactive column is left as it is.  Slots fill from the
+    // lowest free one, so a batch with few systems in flight moves few columns.
     const int k = cap_;
+    int nact = -1;
+    if (!Q_ && !rt().capturing) {
+      nact = 0;
+      for (int j = 0; j < cap_; j++)
+        if (st_[j].m.active) nact = j + 1;
+    }
     Mvec<T>* v = cur(V);
     Mvec<T>* w = cur(W);
     const int ph = (int)(phase_ % 3);
-    op_.apply_shifted_dev(d_shift_, v, w, d_dots_, &d_st_[0].sc[ph], sizeof(MinresState) / sizeof(double));
+    op_.apply_shifted_dev(d_shift_, v, w, d_dots_, &d_st_[0].sc[ph], sizeof(MinresState) / sizeof(double), nact);
     const bool fuse = Q_ && Q_->nvec <= kFusedProjCols;  // w -= Q*C inside the Lanczos update
     if (Q_) {
       gemm_gram<T>(T(1), Q_, w, T(0), C_->get(), false);
@@ -492,7 +504,7 @@ class MinresBatch {
     krylov_minres_step_dev<T>(v->nrows, k, cap_, v->dev(), cur(VOLD)->dev(), w->dev(), cur(D1)->dev(),
                               cur(D2)->dev(), cur(XS)->dev(), d_dots_, d_st_, d_work_, fuse ? Q_->dev() : nullptr,
                               fuse ? Q_->lda : 0, fuse ? Q_->nvec : 0, fuse ? C_->get()->dev() : nullptr,
-                              fuse ? C_->get()->root->ld : 0, ph);
+                              fuse ? C_->get()->root->ld : 0, ph, nact);
     advance_roles();
   }
   // after a device iteration: w (unnormalised) is the next v, v the next

@@ -40,6 +40,7 @@ struct SpmvArgs {
   T* __restrict__ u;
   int64_t ldu;
   int ncols;  // block columns handled by this launch
+  int nact;   // lanes of columns >= nact skip their loads and stores (same kernel, grid and sums)
   T alpha, beta, gamma, delta;
   int flags;
   int blocks_per_cu;  // host side: persistent grid size per CU (0 = as many as fit)
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   const int q = lane % LPR;
   const int rw = lane / LPR;
   const int c0 = q * VPL;
-  const bool colok = c0 < a.ncols;
+  const bool colok = c0 < a.nact;
   const int C = 1 << a.clog2;
   const int flags = a.flags;
 
@@ -341,6 +342,7 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   for (int cb = 0; cb < nvec; cb += maxcols) {
     const int nc = std::min(maxcols, nvec - cb);
     a.ncols = nc;
+    a.nact = req.ncols_active >= 0 ? std::max(0, std::min(nc, req.ncols_active - cb)) : nc;
     a.x = x->dev() + cb;
     a.y = y->dev() + cb;
     a.u = U ? U->dev() + cb : nullptr;
