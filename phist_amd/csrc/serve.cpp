@@ -2,7 +2,9 @@
 //
 // Per (matrix, precision) one MinresBatch with `slots` columns; queued
 // shifted systems fill free slots in arrival order, every step advances each
-// batch by one MINRES iteration and harvests the systems that finished
+// batch by one MINRES iteration (while eigenvalue requests run, by as many
+// as fit in the time their outer iterations took in the last step, at most
+// PHIST_AMD_SERVE_MAX_REPS) and harvests the systems that finished
 // (their x is copied into the request's X column and the slot is released
 // for the next system).  Eigenvalue requests hold a resumable block
 // Jacobi-Davidson and advance one outer iteration per step.  Admission is
@@ -123,16 +125,28 @@ class Server {
 
   int step() {
     admit();
-    // one MINRES iteration per batch, then harvest finished systems
+    // MINRES iterations per batch, then harvest finished systems: one, or, while eigenvalue requests
+    // run, as many as took the time of their outer iterations in the last step (at most
+    // serve_max_reps), so that
+    // shifted solves do not advance at the pace of the eigensolver's steps
+    int reps = 1;
+    if (!running_jd_.empty() && t_jd_ > 0.0 && t_iter_ > 0.0)
+      reps = (int)std::max(1.0, std::min((double)rt().serve_max_reps, std::floor(t_jd_ / t_iter_)));
+    const double t0 = now_s();
+    int iters = 0;
     for (auto& b : batches_) {
-      if (b->n_used() == 0) continue;
-      if (b->fp32)
-        b->mb32->iterate();
-      else
-        b->mb64->iterate();
-      harvest(*b);
+      for (int rep = 0; rep < reps && b->n_used() > 0; rep++) {
+        if (b->fp32)
+          b->mb32->iterate();
+        else
+          b->mb64->iterate();
+        iters++;
+        harvest(*b);
+      }
     }
+    if (iters > 0) t_iter_ = (now_s() - t0) / iters;  // per iteration of one batch, this step
     // one outer iteration per running eigenvalue request
+    const double t1 = now_s();
     for (size_t i = 0; i < running_jd_.size();) {
       Request* r = running_jd_[i];
       bool fin = false;
@@ -158,6 +172,7 @@ class Server {
         i++;
       }
     }
+    t_jd_ = running_jd_.empty() ? 0.0 : now_s() - t1;
     // drop idle batches (their work space returns to the budget)
     for (size_t i = 0; i < batches_.size();) {
       if (batches_[i]->n_used() == 0 && !queued_for(batches_[i]->A, batches_[i]->fp32)) {
@@ -311,6 +326,7 @@ class Server {
   std::deque<Request*> queue_;
   std::vector<std::unique_ptr<Batch>> batches_;
   std::vector<Request*> running_jd_;
+  double t_jd_ = 0.0, t_iter_ = 0.0;  // last step: time of the eigensolver steps, of one batch iteration
 };
 
 }  // namespace
