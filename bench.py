@@ -574,6 +574,7 @@ def main():
     ap.add_argument("--rate", type=float, default=40.0, help="--op serve: mean request arrivals per second")
     ap.add_argument("--duration", type=float, default=20.0, help="--op serve: seconds of arrivals")
     ap.add_argument("--starts", type=int, default=5, help="--op jada: start vectors averaged per precision")
+    ap.add_argument("--max-iter", type=int, default=None, help="--op minres: iteration limit (default 3000)")
     ap.add_argument("--problem", default="xxz24",
                     help="--op jada: xxz<L> (S_z=0 spin chain, 16 smallest) or lap<n> (n^3 Laplacian, 8 smallest)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -598,7 +599,7 @@ def main():
     if args.op == "jada":
         return jada_bench(be, args)
     if args.op == "minres":
-        return minres_bench(be, args)
+        return minres_bench(be, args, max_iter=args.max_iter or 3000)
     if args.op == "serve":
         return serve_bench(be, args)
     if args.op == "carp":
