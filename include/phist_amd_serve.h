@@ -29,6 +29,9 @@ typedef struct phist_amd_server_s* phist_amd_server_ptr;
 #define PHIST_AMD_REQ_RUNNING 1
 #define PHIST_AMD_REQ_DONE 2
 #define PHIST_AMD_REQ_DONE_UNCONVERGED 3
+#define PHIST_AMD_REQ_CANCELLED 4
+/* a request that failed reports its error code (< 0) as its state, e.g.
+   -44 when its work space can never fit the budget */
 
 /* hbm_budget_bytes bounds the work space of admitted requests (solver
    blocks, not the caller's matrices and vectors); slots = systems per MINRES
@@ -43,16 +46,26 @@ void phist_amd_server_submit_solve(phist_amd_server_ptr srv, phist_Dconst_sparse
                                    const double shifts[], phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, double tol,
                                    int maxIter, int fp32, int64_t* id, int* iflag);
 
-/* phist_Dsubspacejada on a matrix of this library (same restrictions);
-   Q, R, ev, resNorm as there; fp32 != 0 solves the correction equations in
-   FP32. The options are copied. */
+/* phist_Dsubspacejada on a matrix of this library (same restrictions: options
+   it does not support fail here with -99); Q, R, ev, resNorm as there;
+   fp32 != 0 solves the correction equations in FP32. The options are copied. */
 void phist_amd_server_submit_jada(phist_amd_server_ptr srv, phist_Dconst_sparseMat_ptr A, const phist_jadaOpts* opts,
                                   int fp32, phist_Dmvec_ptr Q, phist_DsdMat_ptr R, double* ev, double* resNorm,
                                   int64_t* id, int* iflag);
 
-/* admit queued work, then one MINRES iteration of every batch and one outer
-   iteration of every running eigenvalue request; pending = requests not yet
-   done */
+/* admit queued work, then advance every MINRES batch and every running
+   eigenvalue request; pending = requests not yet done.  An eigenvalue request
+   advances one outer iteration per step.  A batch advances by at least one
+   MINRES iteration per step: by more while eigenvalue requests run (as many
+   as fit in the time their outer iterations took in the previous step, at
+   most PHIST_AMD_SERVE_MAX_REPS, default 4), and each of those may queue
+   several iterations on the device (PHIST_AMD_MINRES_ITERS_PER_CALL; default
+   2 for blocks of at most 4M elements, else 1).  The repeat count comes from
+   wall-clock timings, so how far a solve advances per step is not
+   deterministic; the results do not depend on it (a converged system stops
+   updating).  Admission is first come, first served: a request waiting for
+   memory holds back the ones behind it, and batches with no system in
+   flight are dropped to make room for it. */
 void phist_amd_server_step(phist_amd_server_ptr srv, int* pending, int* iflag);
 
 /* state of a request and, once done, its latency (seconds from submission to
@@ -61,7 +74,13 @@ void phist_amd_server_step(phist_amd_server_ptr srv, int* pending, int* iflag);
 void phist_amd_server_status(phist_amd_server_ptr srv, int64_t id, int* state, double* latency_s, int* iters,
                              int* iflag);
 
-/* forget a finished request (its buffers may then be reused) */
+/* withdraw a queued or running request (state PHIST_AMD_REQ_CANCELLED):
+   its systems leave their slots, an eigenvalue request frees its work space;
+   X, Q, R, ev, resNorm hold whatever was written so far.  A finished request
+   is left unchanged.  No counterpart in the reference. */
+void phist_amd_server_cancel(phist_amd_server_ptr srv, int64_t id, int* iflag);
+
+/* forget a finished or cancelled request (its buffers may then be reused) */
 void phist_amd_server_release(phist_amd_server_ptr srv, int64_t id, int* iflag);
 
 #ifdef __cplusplus

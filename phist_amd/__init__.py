@@ -552,7 +552,7 @@ class CarpWork:
 
 class Server:
     """phist_amd_server_*: continuous-batching loop for shifted solves and JD requests (FP64 interface)"""
-    QUEUED, RUNNING, DONE, DONE_UNCONVERGED = 0, 1, 2, 3
+    QUEUED, RUNNING, DONE, DONE_UNCONVERGED, CANCELLED = 0, 1, 2, 3, 4
 
     def __init__(self, be, budget_bytes, slots):
         assert be.p == "D"
@@ -580,22 +580,26 @@ class Server:
         return rid.value
 
     def submit_jada(self, A, num_eigs, which="SR", tol=1e-8, block_size=4, min_bas=None, max_bas=None,
-                    max_iters=300, inner_max_iters=10, fp32=False):
-        """returns (id, Q, R, ev, resNorm); Q/R are Mvec/SdMat owned by the caller"""
+                    max_iters=300, inner_max_iters=10, fp32=False, inner="MINRES"):
+        """returns (id, Q, R, ev, resNorm); Q/R are Mvec/SdMat owned by the caller.
+        `which` is a key of EIGSORT or a raw phist_EigSort value"""
         be = self.be
         opts = JadaOpts()
         self.L.phist_jadaOpts_setDefaults(ctypes.byref(opts))
-        opts.numEigs, opts.which, opts.convTol = num_eigs, EIGSORT[which], tol
+        opts.numEigs, opts.which, opts.convTol = num_eigs, EIGSORT.get(which, which), tol
         opts.symmetry, opts.blockSize = HERMITIAN, block_size
         opts.minBas = min_bas if min_bas is not None else max(10, num_eigs + block_size)
         opts.maxBas = max_bas if max_bas is not None else opts.minBas + 2 * block_size + 10
-        opts.maxIters, opts.innerSolvType, opts.innerSolvMaxIters = max_iters, LINSOLV["MINRES"], inner_max_iters
+        opts.maxIters, opts.innerSolvType, opts.innerSolvMaxIters = max_iters, LINSOLV[inner], inner_max_iters
         nq = num_eigs + block_size - 1
         Q, R = be.mvec(A.domain_map, nq), be.sdmat(nq, nq)
         ev, res = np.zeros(nq), np.zeros(num_eigs)
         rid, fl = ctypes.c_int64(0), ctypes.c_int(0)
         self.L.phist_amd_server_submit_jada(self.h, A.h, ctypes.byref(opts), ctypes.c_int(int(fp32)), Q.h, R.h,
                                             _ptr(ev), _ptr(res), ctypes.byref(rid), ctypes.byref(fl))
+        if fl.value < 0:
+            Q.delete()
+            R.delete()
         self._chk("server_submit_jada", fl)
         self._keep[rid.value] = (opts, ev, res)
         return rid.value, Q, R, ev, res
@@ -612,6 +616,11 @@ class Server:
                                        ctypes.byref(it), ctypes.byref(fl))
         self._chk("server_status", fl)
         return st.value, lat.value, it.value
+
+    def cancel(self, rid):
+        fl = ctypes.c_int(0)
+        self.L.phist_amd_server_cancel(self.h, ctypes.c_int64(rid), ctypes.byref(fl))
+        self._chk("server_cancel", fl)
 
     def release(self, rid):
         fl = ctypes.c_int(0)

@@ -308,6 +308,15 @@ struct SpmvRequest {
   // (their dots are 0); the launch is the one for all columns, so the
   // other columns' results do not depend on it
   int ncols_active = -1;
+  // Gram epilogue (at most 32 columns, no other epilogue): W'W and V'W of
+  // the new block written to these device arrays (column-major, column
+  // stride *_ld); with no_store the new block itself is not written (y may
+  // then be null)
+  void* gram_ww_dev = nullptr;
+  int gram_ww_ld = 0;
+  void* gram_vw_dev = nullptr;
+  int gram_vw_ld = 0;
+  bool no_store = false;
 };
 
 // limits of the blocked GMRES Gram-Schmidt sweeps (systems per block, basis

@@ -627,6 +627,25 @@ void subspacejada(const Operator<T>& A, const Operator<float>* A32, const phist_
 
 }  // namespace
 
+// The option checks of phist_Xsubspacejada, shared by the direct call and
+// the service loop: every combination outside the symmetric standard
+// problem with MINRES, GMRES or no correction solver returns -99
+// (PHIST_NOT_IMPLEMENTED) instead of running something else.
+void jada_validate(const phist_jadaOpts& opts) {
+  PHX_REQUIRE(opts.how == phist_STANDARD, PHIST_NOT_IMPLEMENTED, "only standard Ritz extraction");
+  PHX_REQUIRE(opts.symmetry == phist_HERMITIAN || opts.symmetry == phist_COMPLEX_SYMMETRIC, PHIST_NOT_IMPLEMENTED,
+              "only symmetric problems (opts.symmetry HERMITIAN)");
+  PHX_REQUIRE(opts.which == phist_LM || opts.which == phist_SM || opts.which == phist_LR || opts.which == phist_SR,
+              PHIST_NOT_IMPLEMENTED, "eigenvalue selection %d not supported", (int)opts.which);
+  PHX_REQUIRE(opts.innerSolvType == phist_MINRES || opts.innerSolvType == phist_GMRES ||
+                  opts.innerSolvType == phist_NO_LINSOLV,
+              PHIST_NOT_IMPLEMENTED, "inner solver %d not supported", (int)opts.innerSolvType);
+  PHX_REQUIRE(opts.preconOp == nullptr, PHIST_NOT_IMPLEMENTED, "preconditioned correction equations");
+  PHX_REQUIRE(opts.numEigs >= 1 && opts.blockSize >= 1 && opts.minBas >= 1 && opts.maxBas >= opts.minBas,
+              PHIST_INVALID_INPUT, "numEigs %d, blockSize %d, minBas %d, maxBas %d", opts.numEigs, opts.blockSize,
+              opts.minBas, opts.maxBas);
+}
+
 namespace {
 
 template <typename T>
@@ -648,15 +667,7 @@ void run_jada(const typename LinOpT<T>::type* AB_op, const void* B_op, const phi
               T* ev, T* resNorm, int* nConv, int* nIter) {
   PHX_REQUIRE(AB_op != nullptr, PHIST_INVALID_INPUT, "no operator");
   PHX_REQUIRE(B_op == nullptr, PHIST_NOT_IMPLEMENTED, "generalized problems (B_op) are not supported");
-  PHX_REQUIRE(opts.how == phist_STANDARD, PHIST_NOT_IMPLEMENTED, "only standard Ritz extraction");
-  PHX_REQUIRE(opts.symmetry == phist_HERMITIAN || opts.symmetry == phist_COMPLEX_SYMMETRIC, PHIST_NOT_IMPLEMENTED,
-              "only symmetric problems (opts.symmetry HERMITIAN)");
-  PHX_REQUIRE(opts.which == phist_LM || opts.which == phist_SM || opts.which == phist_LR || opts.which == phist_SR,
-              PHIST_NOT_IMPLEMENTED, "eigenvalue selection %d not supported", (int)opts.which);
-  PHX_REQUIRE(opts.innerSolvType == phist_MINRES || opts.innerSolvType == phist_GMRES ||
-                  opts.innerSolvType == phist_NO_LINSOLV,
-              PHIST_NOT_IMPLEMENTED, "inner solver %d not supported", (int)opts.innerSolvType);
-  PHX_REQUIRE(opts.preconOp == nullptr, PHIST_NOT_IMPLEMENTED, "preconditioned correction equations");
+  jada_validate(opts);
   Operator<T> op;
   op.A = own_matrix<T>(AB_op);
   if (!op.A) op.user = AB_op;
@@ -682,10 +693,7 @@ struct JadaHandle {
 
 JadaHandle* jada_start(const SparseMat<double>* A, bool fp32, const phist_jadaOpts& opts, Mvec<double>* Q,
                        SdMat<double>* R, double* ev, double* resNorm) {
-  PHX_REQUIRE(opts.symmetry == phist_HERMITIAN || opts.symmetry == phist_COMPLEX_SYMMETRIC, PHIST_NOT_IMPLEMENTED,
-              "only symmetric problems (opts.symmetry HERMITIAN)");
-  PHX_REQUIRE(opts.how == phist_STANDARD && opts.preconOp == nullptr, PHIST_NOT_IMPLEMENTED,
-              "standard Ritz values without preconditioner only");
+  jada_validate(opts);
   std::unique_ptr<JadaHandle> h(new JadaHandle());
   h->op.A = A;
   const Operator<float>* lowered = nullptr;

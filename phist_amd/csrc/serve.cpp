@@ -8,7 +8,10 @@
 // (their x is copied into the request's X column and the slot is released
 // for the next system).  Eigenvalue requests hold a resumable block
 // Jacobi-Davidson and advance one outer iteration per step.  Admission is
-// first-come first-served against a byte budget for solver work space.
+// first-come first-served against a byte budget for solver work space; a
+// request whose work space can never fit fails with -44, and idle batches
+// give their memory back when the queue head waits for it.  Queued or
+// running requests can be cancelled.
 #include "phist_amd_serve.h"
 #include "solvers.hpp"
 #include <chrono>
@@ -104,6 +107,7 @@ class Server {
   int64_t submit_jada(const SparseMat<double>* A, const phist_jadaOpts& opts, bool fp32, Mvec<double>* Q,
                       SdMat<double>* R, double* ev, double* res) {
     PHX_REQUIRE(A && Q && R && ev && res, PHIST_INVALID_INPUT, "bad eigenvalue request");
+    jada_validate(opts);  // unsupported options fail at submission, as phist_Dsubspacejada fails
     auto r = std::make_unique<Request>();
     r->id = ++last_id_;
     r->jada = true;
@@ -197,6 +201,43 @@ class Server {
     *iters = r.iters;
   }
 
+  // Withdraws a queued or running request: its systems leave their slots
+  // (which the next step refills), a running eigenvalue request frees its
+  // work space; its outputs are left as they are.  A finished request is
+  // not changed.
+  void cancel(int64_t id) {
+    auto it = reqs_.find(id);
+    PHX_REQUIRE(it != reqs_.end(), PHIST_INVALID_INPUT, "unknown request %lld", (long long)id);
+    Request* r = it->second.get();
+    if (r->state != PHIST_AMD_REQ_QUEUED && r->state != PHIST_AMD_REQ_RUNNING) return;
+    for (auto q = queue_.begin(); q != queue_.end();)
+      q = (*q == r) ? queue_.erase(q) : q + 1;
+    if (r->jada) {
+      if (r->jd) {
+        jada_free(r->jd);
+        r->jd = nullptr;
+        in_use_ -= r->bytes;
+        for (size_t i = 0; i < running_jd_.size(); i++)
+          if (running_jd_[i] == r) {
+            running_jd_.erase(running_jd_.begin() + i);
+            break;
+          }
+      }
+    } else {
+      for (auto& b : batches_)
+        for (int j = 0; j < slots_; j++)
+          if (b->owner[j].first == r) {
+            if (b->fp32)
+              b->mb32->release(j);
+            else
+              b->mb64->release(j);
+            b->owner[j] = {nullptr, 0};
+          }
+    }
+    r->state = PHIST_AMD_REQ_CANCELLED;
+    r->t_done = now_s();
+  }
+
   void release(int64_t id) {
     auto it = reqs_.find(id);
     PHX_REQUIRE(it != reqs_.end(), PHIST_INVALID_INPUT, "unknown request %lld", (long long)id);
@@ -219,9 +260,31 @@ class Server {
     return nullptr;
   }
 
+  // work space of a MINRES batch: its six blocks of slots columns (an FP32
+  // copy of the matrix belongs to the matrix, i.e. to the caller's memory)
+  int64_t batch_bytes(const SparseMat<double>* A, bool fp32) const {
+    return (int64_t)(fp32 ? 4 : 8) * A->nrows * 6 * slots_;
+  }
+
+  // Drops batches with no system in flight until `need` more bytes fit the
+  // budget.  Called for the queue head when it waits for memory: an idle
+  // batch is kept at the end of a step while requests for it are queued,
+  // but requests behind the head cannot use it before the head runs, so
+  // holding it would block the head (and everything behind it) for good.
+  void release_idle_batches(int64_t need) {
+    for (size_t i = 0; i < batches_.size() && in_use_ + need > budget_;) {
+      if (batches_[i]->n_used() == 0) {
+        in_use_ -= batches_[i]->bytes;
+        batches_.erase(batches_.begin() + i);
+      } else {
+        i++;
+      }
+    }
+  }
+
   Batch* open_batch(const SparseMat<double>* A, bool fp32) {
-    // the solver blocks; an FP32 copy of the matrix belongs to the matrix (caller memory)
-    const int64_t bytes = (int64_t)(fp32 ? 4 : 8) * A->nrows * 6 * slots_;
+    const int64_t bytes = batch_bytes(A, fp32);
+    if (in_use_ + bytes > budget_) release_idle_batches(bytes);
     if (in_use_ + bytes > budget_) return nullptr;
     auto b = std::make_unique<Batch>();
     b->A = A;
@@ -252,6 +315,7 @@ class Server {
           it = queue_.erase(it);
           continue;
         }
+        if (in_use_ + r->bytes > budget_) release_idle_batches(r->bytes);
         if (in_use_ + r->bytes > budget_) break;  // wait for memory, keep the order
         try {
           r->jd = jada_start(r->A, r->fp32, r->opts, r->Q, r->R, r->ev, r->res);
@@ -265,6 +329,11 @@ class Server {
         continue;
       }
       Batch* b = find_batch(r->A, r->fp32);
+      if (!b && batch_bytes(r->A, r->fp32) > budget_) {  // can never fit
+        fail(r, PHIST_MEM_ALLOC_FAILED);
+        it = queue_.erase(it);
+        continue;
+      }
       if (!b) b = open_batch(r->A, r->fp32);
       if (!b) break;  // no memory for a new batch: keep the order
       int slot;
@@ -395,6 +464,14 @@ void phist_amd_server_status(phist_amd_server_ptr srv, int64_t id, int* state, d
   guard(iflag, "phist_amd_server_status", [&] {
     PHX_REQUIRE(srv, PHIST_INVALID_INPUT, "null server");
     srv->s->status(id, state, latency_s, iters);
+  });
+}
+
+void phist_amd_server_cancel(phist_amd_server_ptr srv, int64_t id, int* iflag) {
+  *iflag = 0;
+  guard(iflag, "phist_amd_server_cancel", [&] {
+    PHX_REQUIRE(srv, PHIST_INVALID_INPUT, "null server");
+    srv->s->cancel(id);
   });
 }
 

@@ -808,6 +808,8 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
 // (defined in jada.cpp): start builds the start-up basis, step runs one outer
 // iteration (true when finished), finish writes the outputs
 struct JadaHandle;
+// throws -99 for options outside what subspacejada supports, -55 for inconsistent sizes
+void jada_validate(const phist_jadaOpts& opts);
 JadaHandle* jada_start(const SparseMat<double>* A, bool fp32, const phist_jadaOpts& opts, Mvec<double>* Q,
                        SdMat<double>* R, double* ev, double* resNorm);
 bool jada_step(JadaHandle* h);

@@ -8,7 +8,8 @@
 // spmvm_kernels_nt.c:103-126), plus optional epilogues fused into the same
 // pass: column dots <y_new, y_new> and <x, y_new> (fused_spmv_mvdot,
 // phist_kernels_fused_decl.h:56) and U = gamma*y_new + delta*U
-// (fused_spmv_mvdot_mvadd, :77).
+// (fused_spmv_mvdot_mvadd, :77), or the Gram matrices W'W and V'W of the
+// new block (fused_spmv_mvTmv, :68) on the matrix cores.
 //
 // Mapping onto CDNA4: a row is owned by LPR lanes of one wave64, each lane
 // holding VPL consecutive block columns (16-byte vectors where the layout
@@ -62,7 +63,34 @@ inline int spmv_table_blocks_per_cu(double nnz_per_row, int nvec) {
   return 6;
 }
 
-enum : int { F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16 };
+enum : int { F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16, F_GWW = 32, F_GVW = 64, F_NOSTORE = 128 };
+
+// v_mfma_{f64,f32}_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// and holds four results of the 16x16 product in column l%16; the row of
+// result register r is l/16 + 4r for f64 and 4*(l/16) + r for f32 (the
+// layout the Gram kernels of tsgemm.hip use).
+template <typename T>
+struct SpmvMfma;
+template <>
+struct SpmvMfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct SpmvMfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// orders the LDS accesses of one wave (its lanes exchange data through LDS;
+// a wave's LDS instructions complete in order)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Broadcast of lane `src` (0 <= src < LPR) of every LPR-lane group: DPP
 // quad permutes for groups of 2 and 4 lanes (VALU, no LDS traffic), a
@@ -101,7 +129,15 @@ __device__ inline T group_bcast_t(T v, int src, int lane) {
 // cooperatively (lane q takes slot j = jb*LPR + q, coalesced over the rows
 // of the chunk) and share them by group broadcast, so every index is read
 // from memory once.  KB loads per lane are in flight per batch.
-template <typename T, int VPL, int LPR, bool XSC>
+//
+// GR > 0 (fused_spmv_mvTmv, at most 32 columns, GR = 16-column tiles): after
+// each tile the wave writes its rows of the new W (and of V) to LDS and
+// reads them back in the matrix-core operand layout, four rows per MFMA
+// step, accumulating W'W (upper tiles, mirrored at the end) and V'W in
+// registers; the wave tiles are added in LDS in wave order and the
+// workgroup tiles by a second fixed-order pass, so the sums are
+// deterministic.  W itself is not stored with F_NOSTORE (W = NULL).
+template <typename T, int VPL, int LPR, bool XSC, int GR>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   constexpr int RPW = 64 / LPR;
   constexpr int RPB = (kBlock / 64) * RPW;
@@ -126,6 +162,15 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   T dww[VPL], dvw[VPL];
 #pragma unroll
   for (int e = 0; e < VPL; e++) dww[e] = dvw[e] = T(0);
+  using GM = SpmvMfma<T>;
+  constexpr int TG = GR > 0 ? GR : 1;  // 16-column tiles of the Gram matrices
+  constexpr int GW = LPR * VPL;        // row stride of the exchange tiles
+  __shared__ T gex[GR > 0 ? kBlock / 64 : 1][2][GR > 0 ? 64 * VPL : 1];
+  typename GM::acc_t gww[TG][TG], gvw[TG][TG];
+#pragma unroll
+  for (int i = 0; i < TG; i++)
+#pragma unroll
+    for (int j = 0; j < TG; j++) gww[i][j] = gvw[i][j] = typename GM::acc_t{0, 0, 0, 0};
 
   const TileWalk tw = xcd_tiles(a.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
@@ -147,7 +192,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
     T acc[VPL], xo[VPL];
 #pragma unroll
     for (int e = 0; e < VPL; e++) xo[e] = T(0);
-    if (live && colok && (flags & (F_SHIFT | F_DVW))) vload<T, VPL>(xb + (int64_t)row * a.ldx, xo);
+    if (live && colok && (flags & (F_SHIFT | F_DVW | F_GVW))) vload<T, VPL>(xb + (int64_t)row * a.ldx, xo);
 #pragma unroll
     for (int e = 0; e < VPL; e++) acc[e] = (flags & F_SHIFT) ? sh[e] * xo[e] : T(0);
 
@@ -188,42 +233,113 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
           }
       }
     }
-    if (!live || !colok) continue;
-    if constexpr (XSC) {  // (A + s)(x .* f) = ((A + s) x) .* f, one product per result
+    T yn[VPL];
 #pragma unroll
-      for (int e = 0; e < VPL; e++) {
-        acc[e] *= xsc[e];
-        xo[e] *= xsc[e];
+    for (int e = 0; e < VPL; e++) yn[e] = T(0);
+    if (live && colok) {
+      if constexpr (XSC) {  // (A + s)(x .* f) = ((A + s) x) .* f, one product per result
+#pragma unroll
+        for (int e = 0; e < VPL; e++) {
+          acc[e] *= xsc[e];
+          xo[e] *= xsc[e];
+        }
+      }
+      T* yp = a.y + (int64_t)row * a.ldy + c0;
+      if (flags & F_BETA) {
+        T yo[VPL];
+        vload<T, VPL>(yp, yo);
+#pragma unroll
+        for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e] + a.beta * yo[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e];
+      }
+      if (!(flags & F_NOSTORE)) vstore<T, VPL>(yp, yn);
+      if (flags & F_DWW) {
+#pragma unroll
+        for (int e = 0; e < VPL; e++) dww[e] = fma(yn[e], yn[e], dww[e]);
+      }
+      if (flags & F_DVW) {
+#pragma unroll
+        for (int e = 0; e < VPL; e++) dvw[e] = fma(xo[e], yn[e], dvw[e]);
+      }
+      if (flags & F_AXPY) {
+        T* up = a.u + (int64_t)row * a.ldu + c0;
+        T uo[VPL], un[VPL];
+        vload<T, VPL>(up, uo);
+#pragma unroll
+        for (int e = 0; e < VPL; e++) un[e] = a.gamma * yn[e] + a.delta * uo[e];
+        vstore<T, VPL>(up, un);
       }
     }
+    if constexpr (GR > 0) {
+      // rows of the wave (dead rows and inactive columns are zero) through LDS into the MFMA layout
+      T* wt = gex[wv][0];
+      T* vt = gex[wv][1];
+#pragma unroll
+      for (int e = 0; e < VPL; e++) {
+        wt[rw * GW + c0 + e] = yn[e];
+        vt[rw * GW + c0 + e] = (live && colok) ? xo[e] : T(0);
+      }
+      wave_lds_sync();
+      constexpr int KS = (RPW + 3) / 4;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int r = 4 * ks + (lane >> 4);
+        T aw[TG], av[TG];
+#pragma unroll
+        for (int tm = 0; tm < TG; tm++) {
+          const int col = tm * 16 + (lane & 15);
+          const bool ok = r < RPW && col < GW;
+          aw[tm] = ok ? wt[r * GW + col] : T(0);
+          av[tm] = ok ? vt[r * GW + col] : T(0);
+        }
+#pragma unroll
+        for (int tm = 0; tm < TG; tm++)
+#pragma unroll
+          for (int tk = 0; tk < TG; tk++) {
+            if ((flags & F_GWW) && tm <= tk) gww[tm][tk] = GM::op(aw[tm], aw[tk], gww[tm][tk]);
+            if (flags & F_GVW) gvw[tm][tk] = GM::op(av[tm], aw[tk], gvw[tm][tk]);
+          }
+      }
+      wave_lds_sync();  // the next tile overwrites the exchange tiles
+    }
+  }
 
-    T yn[VPL];
-    T* yp = a.y + (int64_t)row * a.ldy + c0;
-    if (flags & F_BETA) {
-      T yo[VPL];
-      vload<T, VPL>(yp, yo);
+  if constexpr (GR > 0) {
+    // S = [W'W, V'W] of this workgroup: the four waves' tiles added in LDS in
+    // wave order, then stored as partials[blockIdx.x][n*n] and
+    // partials[gridDim.x + blockIdx.x][n*n] (column-major n x n)
+    __shared__ T gsum[2][(16 * TG) * (16 * TG)];
+    const int n = a.ncols;
+#pragma unroll 1
+    for (int w = 0; w < kBlock / 64; w++) {
+      if (wv == w) {
 #pragma unroll
-      for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e] + a.beta * yo[e];
-    } else {
+        for (int tm = 0; tm < TG; tm++)
 #pragma unroll
-      for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e];
+          for (int tk = 0; tk < TG; tk++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int i = tm * 16 + GM::row(lane, r), j = tk * 16 + (lane & 15);
+              if (i >= n || j >= n) continue;
+              if ((flags & F_GWW) && tm <= tk) {
+                const T v = gww[tm][tk][r];
+                gsum[0][j * n + i] = w == 0 ? v : gsum[0][j * n + i] + v;
+                if (tm < tk) gsum[0][i * n + j] = w == 0 ? v : gsum[0][i * n + j] + v;
+              }
+              if (flags & F_GVW) {
+                const T v = gvw[tm][tk][r];
+                gsum[1][j * n + i] = w == 0 ? v : gsum[1][j * n + i] + v;
+              }
+            }
+      }
+      __syncthreads();
     }
-    vstore<T, VPL>(yp, yn);
-    if (flags & F_DWW) {
-#pragma unroll
-      for (int e = 0; e < VPL; e++) dww[e] = fma(yn[e], yn[e], dww[e]);
-    }
-    if (flags & F_DVW) {
-#pragma unroll
-      for (int e = 0; e < VPL; e++) dvw[e] = fma(xo[e], yn[e], dvw[e]);
-    }
-    if (flags & F_AXPY) {
-      T* up = a.u + (int64_t)row * a.ldu + c0;
-      T uo[VPL], un[VPL];
-      vload<T, VPL>(up, uo);
-#pragma unroll
-      for (int e = 0; e < VPL; e++) un[e] = a.gamma * yn[e] + a.delta * uo[e];
-      vstore<T, VPL>(up, un);
+    const int nn = n * n;
+    for (int o = threadIdx.x; o < nn; o += kBlock) {
+      a.partials[(int64_t)blockIdx.x * nn + o] = gsum[0][o];
+      a.partials[((int64_t)gridDim.x + blockIdx.x) * nn + o] = gsum[1][o];
     }
   }
 
@@ -255,13 +371,15 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 
 namespace {
 
-template <typename T, int VPL, int LPR, bool XSC>
+template <typename T, int VPL, int LPR, bool XSC, int GR = 0>
 int launch_one(SpmvArgs<T>& a) {
-  auto kern = k_sell_spmv<T, VPL, LPR, XSC>;
+  auto kern = k_sell_spmv<T, VPL, LPR, XSC, GR>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   a.ntiles = (a.nslots + RPB - 1) / RPB;
   const int grid = persistent_blocks(kern, 0, a.ntiles, a.blocks_per_cu);
-  if (a.flags & (F_DWW | F_DVW))
+  if (GR > 0)
+    a.partials = (T*)device_work(sizeof(T) * (size_t)2 * grid * a.ncols * a.ncols);
+  else if (a.flags & (F_DWW | F_DVW))
     a.partials = (T*)device_work(sizeof(T) * ((size_t)grid * 2 * a.ncols + 2 * a.ncols));
   ktimer_begin();
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, a);
@@ -288,6 +406,32 @@ int launch_xsc(SpmvArgs<T>& a, int lpr) {
   return a.xscale ? launch_lpr<T, VPL, true>(a, lpr) : launch_lpr<T, VPL, false>(a, lpr);
 }
 
+// the Gram-epilogue variants: at most 32 columns (lane groups of at most 32 lanes)
+template <typename T, int VPL, int GR>
+int launch_gram_lpr(SpmvArgs<T>& a, int lpr) {
+  switch (lpr) {
+    case 1: return launch_one<T, VPL, 1, false, GR>(a);
+    case 2: return launch_one<T, VPL, 2, false, GR>(a);
+    case 4: return launch_one<T, VPL, 4, false, GR>(a);
+    case 8: return launch_one<T, VPL, 8, false, GR>(a);
+    case 16: return launch_one<T, VPL, 16, false, GR>(a);
+    default: return launch_one<T, VPL, 32, false, GR>(a);
+  }
+}
+template <typename T, int VPL>
+int launch_gram(SpmvArgs<T>& a, int lpr) {
+  return a.ncols <= 16 ? launch_gram_lpr<T, VPL, 1>(a, lpr) : launch_gram_lpr<T, VPL, 2>(a, lpr);
+}
+
+// C(0:n, 0:n) (column stride ldc) = sum over the nblocks partial n x n tiles, in a fixed order
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_gram_out(const T* __restrict__ partials, int nblocks, int n,
+                                                          T* __restrict__ C, int ldc) {
+  const T s = block_sum16(partials, nblocks, n * n);
+  const int o = blockIdx.x * 16 + threadIdx.x;
+  if (threadIdx.x < 16 && o < n * n) C[(int64_t)(o / n) * ldc + o % n] = s;
+}
+
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
@@ -296,21 +440,30 @@ template <typename T>
 void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mvec<T>* x, T beta, Mvec<T>* y,
                  const SpmvRequest& req, T* dot_ww_host, T* dot_vw_host, T gamma, T delta, Mvec<T>* U) {
   const int nvec = x->nvec;
-  PHX_REQUIRE(y->nvec == nvec, PHIST_FUNCTIONAL_ERROR, "x has %d columns, y has %d", nvec, y->nvec);
-  PHX_REQUIRE(x->nrows == A->ncols && y->nrows == A->nrows, PHIST_INVALID_INPUT,
-              "shape mismatch: A is %lld x %lld, x has %lld rows, y has %lld rows", (long long)A->nrows,
-              (long long)A->ncols, (long long)x->nrows, (long long)y->nrows);
+  const bool gram = req.gram_ww_dev || req.gram_vw_dev;
+  PHX_REQUIRE(y || (req.no_store && gram && !req.has_beta && !req.axpy_u && !req.dot_ww && !req.dot_vw),
+              PHIST_INVALID_INPUT, "no output block");
+  PHX_REQUIRE(!gram || (nvec <= 32 && A->nrows == A->ncols && !req.dot_ww && !req.dot_vw && !req.axpy_u &&
+                        !req.xscale && req.ncols_active < 0),
+              PHIST_INVALID_INPUT, "Gram epilogue: at most 32 columns of a square matrix, no other epilogue");
+  if (y) {
+    PHX_REQUIRE(y->nvec == nvec, PHIST_FUNCTIONAL_ERROR, "x has %d columns, y has %d", nvec, y->nvec);
+    PHX_REQUIRE(y->nrows == A->nrows, PHIST_INVALID_INPUT, "shape mismatch: A has %lld rows, y has %lld",
+                (long long)A->nrows, (long long)y->nrows);
+  }
+  PHX_REQUIRE(x->nrows == A->ncols, PHIST_INVALID_INPUT, "shape mismatch: A has %lld columns, x has %lld rows",
+              (long long)A->ncols, (long long)x->nrows);
   PHX_REQUIRE(!req.axpy_u || (U && U->nvec == nvec && U->nrows == y->nrows), PHIST_INVALID_INPUT, "bad U block");
   PHX_REQUIRE(!(req.has_shift || req.dot_vw) || A->nrows == A->ncols, PHIST_INVALID_INPUT,
               "shifts and <x,y> dots need a square matrix (A is %lld x %lld)", (long long)A->nrows,
               (long long)A->ncols);
-  if (x->root == y->root)
+  if (y && x->root == y->root)
     PHX_REQUIRE(x->jofs + nvec <= y->jofs || y->jofs + nvec <= x->jofs, PHIST_INVALID_INPUT,
                 "x and y overlap in memory");
 
   constexpr int V16 = 16 / sizeof(T);
-  bool vec_ok = (nvec % V16 == 0) && (x->lda % V16 == 0) && (y->lda % V16 == 0) && aligned16(x->dev()) &&
-                aligned16(y->dev());
+  bool vec_ok = (nvec % V16 == 0) && (x->lda % V16 == 0) && aligned16(x->dev());
+  if (y) vec_ok = vec_ok && (y->lda % V16 == 0) && aligned16(y->dev());
   if (req.axpy_u) vec_ok = vec_ok && (U->lda % V16 == 0) && aligned16(U->dev());
   const int VPL = vec_ok ? V16 : 1;
   const int maxcols = 64 * VPL;
@@ -326,14 +479,15 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   a.clog2 = clog2;
   a.nslots = A->nchunks * A->C;
   a.ldx = x->lda;
-  a.ldy = y->lda;
+  a.ldy = y ? y->lda : 0;
   a.ldu = U ? U->lda : 0;
   a.alpha = alpha;
   a.beta = beta;
   a.gamma = gamma;
   a.delta = delta;
   a.flags = (req.has_shift ? F_SHIFT : 0) | (req.has_beta ? F_BETA : 0) | (req.dot_ww ? F_DWW : 0) |
-            (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0);
+            (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0) | (req.gram_ww_dev ? F_GWW : 0) |
+            (req.gram_vw_dev ? F_GVW : 0) | ((req.no_store || !y) ? F_NOSTORE : 0);
   a.blocks_per_cu = rt().spmv_blocks_per_cu >= 0
                         ? rt().spmv_blocks_per_cu
                         : spmv_table_blocks_per_cu(A->nrows ? (double)A->nnz / (double)A->nrows : 0.0, nvec);
@@ -344,7 +498,7 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
     a.ncols = nc;
     a.nact = req.ncols_active >= 0 ? std::max(0, std::min(nc, req.ncols_active - cb)) : nc;
     a.x = x->dev() + cb;
-    a.y = y->dev() + cb;
+    a.y = y ? y->dev() + cb : nullptr;
     a.u = U ? U->dev() + cb : nullptr;
     a.shift_dev = (req.has_shift && req.shifts_dev) ? static_cast<const T*>(req.shifts_dev) + cb : nullptr;
     for (int c = 0; c < nc; c++) a.shift[c] = (req.has_shift && !a.shift_dev) ? shifts_host[cb + c] : T(0);
@@ -353,6 +507,22 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
     while (lpr < groups) lpr <<= 1;
     a.xscale = req.xscale ? req.xscale + (int64_t)cb * req.xscale_stride : nullptr;
     a.xscale_stride = req.xscale_stride;
+    if (gram) {  // one column block (nvec <= 32)
+      const int grid = VPL == 1 ? launch_gram<T, 1>(a, lpr) : launch_gram<T, V16>(a, lpr);
+      const int nn = nc * nc;
+      if (req.gram_ww_dev) {
+        hipLaunchKernelGGL(k_spmv_gram_out<T>, dim3((nn + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials, grid,
+                           nc, static_cast<T*>(req.gram_ww_dev), req.gram_ww_ld);
+        PHX_HIP(hipGetLastError());
+      }
+      if (req.gram_vw_dev) {
+        hipLaunchKernelGGL(k_spmv_gram_out<T>, dim3((nn + 15) / 16), dim3(kBlock), 0, rt().stream,
+                           a.partials + (size_t)grid * nn, grid, nc, static_cast<T*>(req.gram_vw_dev),
+                           req.gram_vw_ld);
+        PHX_HIP(hipGetLastError());
+      }
+      continue;
+    }
     const int grid = VPL == 1 ? launch_xsc<T, 1>(a, lpr) : launch_xsc<T, V16>(a, lpr);
     if (dots && req.dots_dev) {
       PHX_REQUIRE(cb == 0 && nc == nvec, PHIST_INVALID_INPUT, "device dots need one column block (%d columns)", nvec);

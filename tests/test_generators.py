@@ -11,9 +11,13 @@ Pins (CPU):
 * the oracle's honeycomb tree and the library's generator agree row by row;
   the nearest-neighbour sheet on a torus reproduces the analytic spectrum
   +-|1 + exp(2i k_w) + exp(i (k_l + k_w))| of the brick-wall lattice, the open
-  sheet is bipartite (spectrum symmetric about 0), and the long-range sheet
-  is symmetric.  The long-range values (t2, t3) are pinned only by the
-  restatement cross-check and symmetry (parity partial).
+  sheet is bipartite (spectrum symmetric about 0).  The long-range sheet
+  (second neighbours t2, third neighbours t3 across each hexagon, 3*t3 on the
+  diagonal) equals, entry by entry, a Hamiltonian built here from the
+  nearest-neighbour graph alone (second neighbours = graph distance 2, third
+  neighbours = opposite corners of the graph's 6-cycles), on open, cylinder
+  and torus sheets; on the torus its spectrum is the union of the 2x2 Bloch
+  Hamiltonians H(k) over the cell momenta.
 * the on-site disorder differs from the reference by design (the reference
   draws C rand() in call order, which depends on how rows are visited); the
   library's values are checked for range, determinism and seed dependence.
@@ -170,3 +174,116 @@ def test_honeycomb_disorder():
     assert np.all((d1 >= -0.5) & (d1 < 0.5))    # gamma * (u - 1/2), u in [0, 1)
     assert not np.array_equal(d1, d2)
     assert abs(d1.mean()) < 0.1
+
+
+T1, T2, T3 = -1.0, -0.1 / 2.78, -0.095 / 2.78  # graphene.c:58-60
+
+
+def brickwall_nn(W, L, pbc_w, pbc_l):
+    """nearest-neighbour adjacency of the brick-wall honeycomb, site (l, w) = row l*W + w: every site is
+    bonded to (l, w+-1); a site with l+w odd also to (l+1, w), with l+w even to (l-1, w)"""
+    n = W * L
+    adj = [set() for _ in range(n)]
+
+    def site(l, w):
+        if pbc_w:
+            w %= W
+        if pbc_l:
+            l %= L
+        return l * W + w if 0 <= l < L and 0 <= w < W else None
+
+    for l in range(L):
+        for w in range(W):
+            a = l * W + w
+            for b in (site(l, w + 1), site(l + 1, w) if (l + w) % 2 else None):
+                if b is not None and b != a:
+                    adj[a].add(b)
+                    adj[b].add(a)
+    return adj
+
+
+def honeycomb_from_graph(adj):
+    """t1 on graph neighbours, t2 on graph distance 2, t3 between opposite corners of 6-cycles
+    (paths a-b-c-d with a 3-step return d-e-f-a through distinct vertices), 3*t3 on the diagonal"""
+    n = len(adj)
+    H = np.zeros((n, n))
+    for a in range(n):
+        H[a, a] = 3 * T3
+        for b in adj[a]:
+            H[a, b] = T1
+        d2 = {c for b in adj[a] for c in adj[b]} - adj[a] - {a}
+        for c in d2:
+            H[a, c] = T2
+        opp = set()
+        for b in adj[a]:
+            for c in adj[b] - {a}:
+                for d in adj[c] - {a, b}:
+                    if d in adj[a] or d in d2:
+                        continue
+                    # a 6-cycle a-b-c-d-e-f-a with e, f new vertices
+                    for e in adj[d] - {a, b, c}:
+                        for f in adj[e] - {a, b, c, d}:
+                            if a in adj[f]:
+                                opp.add(d)
+        for d in opp:
+            H[a, d] = T3
+    return H
+
+
+def honeycomb_sheet(W, L, pbc_w, pbc_l):
+    """the long-range sheet from the graph of the infinite lattice: open directions are padded by two
+    sites on both sides (an even shift keeps the sublattice pattern), the shells are found on the padded
+    graph and restricted to the sheet, so a bond exists when its two sites do (graphene.c's edges)"""
+    pw, pl = (0 if pbc_w else 2), (0 if pbc_l else 2)
+    Wp, Lp = W + 2 * pw, L + 2 * pl
+    Hp = honeycomb_from_graph(brickwall_nn(Wp, Lp, pbc_w, pbc_l))
+    idx = np.array([(l + pl) * Wp + (w + pw) for l in range(L) for w in range(W)])
+    return Hp[np.ix_(idx, idx)]
+
+
+@pytest.mark.parametrize("W,L,pbc_w,pbc_l", [(6, 4, 0, 0), (8, 6, 0, 0), (7, 5, 0, 0), (8, 6, 1, 0), (8, 5, 1, 0),
+                                             (6, 6, 0, 1), (7, 4, 0, 1), (8, 6, 1, 1), (10, 8, 1, 1)])
+def test_honeycomb_long_range_from_graph(W, L, pbc_w, pbc_l):
+    """periodic directions need an even extent (the sublattice pattern) and W >= 8 around the w ring
+    (a 6-site ring would be a 6-cycle of its own)"""
+    A = ol.rows_to_dense(ol.honeycomb_rows(W, L, pbc_w, pbc_l, 1), W * L)
+    assert np.array_equal(A, honeycomb_sheet(W, L, pbc_w, pbc_l))
+
+
+def test_honeycomb_long_range_catches_planted_errors():
+    """the graph construction separates the three shells: swapping t2 and t3, or dropping the
+    3*t3 diagonal, changes the matrix"""
+    W, L = 8, 6
+    A = ol.rows_to_dense(ol.honeycomb_rows(W, L, 1, 1, 1), W * L)
+    H = honeycomb_sheet(W, L, 1, 1)
+    swapped = np.where(H == T2, T3, np.where(H == T3, T2, H))
+    assert not np.array_equal(A, swapped)
+    assert not np.array_equal(A, H - 3 * T3 * np.eye(W * L))
+    deg = [(np.sum(H[r] == T1), np.sum(H[r] == T2), np.sum(H[r] == T3)) for r in range(W * L)]
+    assert set(deg) == {(3, 6, 3)}  # honeycomb shells: 3 first, 6 second, 3 third neighbours
+
+
+@pytest.mark.parametrize("W,L", [(8, 6), (10, 8), (12, 4)])
+def test_honeycomb_long_range_torus_bloch(W, L):
+    """torus spectrum of the long-range sheet = eigenvalues of the 2x2 Bloch Hamiltonians
+    H(k)_{ab} = sum over bonds from the a-site of a cell to b-sites of t * exp(i k.d) (d the bond vector,
+    minimum image), k = (2 pi p / L, 2 pi q / W), q < W/2 (cells spanned by (0, 2) and (1, 1))"""
+    A = ol.rows_to_dense(ol.honeycomb_rows(W, L, 1, 1, 1), W * L)
+    got = np.sort(np.linalg.eigvalsh(A))
+    # bonds of the two sites of the cell at the origin: A = (0, 0) (l+w even), B = (0, 1)
+    H = honeycomb_sheet(W, L, 1, 1)
+    sites = [(0, 0), (0, 1)]
+    ev = []
+    for pp, qq in itertools.product(range(L), range(W // 2)):
+        k = np.array([2 * np.pi * pp / L, 2 * np.pi * qq / W])
+        Hk = np.zeros((2, 2), complex)
+        for ia, (la, wa) in enumerate(sites):
+            ra = la * W + wa
+            for rb in np.nonzero(H[ra])[0]:
+                lb, wb = divmod(int(rb), W)
+                dl = (lb - la + L // 2) % L - L // 2
+                dw = (wb - wa + W // 2) % W - W // 2
+                ib = (lb + wb) % 2
+                Hk[ia, ib] += H[ra, rb] * np.exp(1j * (k[0] * dl + k[1] * dw))
+        ev += list(np.linalg.eigvalsh(Hk))
+    assert np.allclose(got, np.sort(ev), atol=1e-10)

@@ -318,3 +318,137 @@ def test_spmv_column_blocks_large(be64, be32, dtype, nvec):
     assert np.all(np.abs(vw - ol.dot(Xf, Wref)) <= 10 * tol * scale)
     for o in (x, y, w, A):
         o.delete()
+
+
+def _p(a):
+    import ctypes
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [1, 4, 8])
+def test_spmv_add_mvec_scalar_shift(be64, be32, dtype, nvec):
+    """sparseMat_times_mvec_add_mvec: y = alpha*(A + shift I)x + beta*y with one scalar shift
+    (kernels_common_impl_def.h:55-64)"""
+    be = be64 if dtype == np.float64 else be32
+    A = be.read_mm(mm("Dsprandn388"))
+    ref = ol.crs_from_mm(mm("Dsprandn388"))
+    rng = np.random.default_rng(nvec + 40)
+    X = rng.uniform(-1, 1, (388, nvec)).astype(dtype)
+    Y0 = rng.uniform(-1, 1, (388, nvec)).astype(dtype)
+    m = be.map(388)
+    x, y = be.mvec(m, nvec, X), be.mvec(m, nvec, Y0)
+    be.call("sparseMat_times_mvec_add_mvec", be.s(0.6), A.h, be.s(-1.7), x.h, be.s(0.4), y.h)
+    want = ol.spmv(ref, 0.6, X.astype(float), 0.4, Y0.astype(float), shifts=np.full(nvec, -1.7))
+    assert colwise_rel(y.get().astype(float), want) <= RTOL[dtype]
+    for o in (x, y, A):
+        o.delete()
+
+
+def _gram_bound(tol, a, b):
+    return 8 * tol * (np.abs(a).T @ np.abs(b))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [1, 3, 4, 8, 16, 17, 24, 32, 40])
+@pytest.mark.parametrize("case", ["W", "W_beta", "W_null", "ww_only", "vw_only", "W_no_gram"])
+def test_fused_spmv_mvTmv(be64, be32, dtype, nvec, case):
+    """fused_spmv_mvTmv (phist_kernels_fused_decl.h:68, kernels_no_fused_spmv_mvTmv.cpp:9-57):
+    W = alpha*A*V + beta*W, WtW = W'W, VtW = V'W; W = NULL means alpha = 1, beta = 0 and W is not
+    returned; WtW / VtW may be NULL.  Up to 32 columns this is one pass (Gram epilogue on the matrix
+    cores), 40 columns take the three-pass path."""
+    be = be64 if dtype == np.float64 else be32
+    A = be.read_mm(mm("DjadaTestMat512"))
+    ref = ol.crs_from_mm(mm("DjadaTestMat512"))
+    n = ref.nrows
+    rng = np.random.default_rng(nvec * 7 + len(case))
+    V = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    W0 = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    alpha, beta = 0.9, (0.35 if case == "W_beta" else 0.0)
+    m = be.map(n)
+    v = be.mvec(m, nvec, V)
+    w = None if case == "W_null" else be.mvec(m, nvec, W0)
+    C0 = rng.uniform(-1, 1, (nvec, nvec))
+    wtw = None if case in ("vw_only", "W_no_gram") else be.sdmat(nvec, nvec, C0)
+    vtw = None if case in ("ww_only", "W_no_gram") else be.sdmat(nvec, nvec, C0)
+    be.call("fused_spmv_mvTmv", be.s(alpha), A.h, v.h, be.s(beta), w.h if w else None, wtw.h if wtw else None,
+            vtw.h if vtw else None)
+    Vf = V.astype(float)
+    a_eff, b_eff = (1.0, 0.0) if w is None else (alpha, beta)
+    Wref = ol.spmv(ref, a_eff, Vf, b_eff, W0.astype(float))
+    tol = RTOL[dtype]
+    if w is not None:
+        Wgot = w.get().astype(float)
+        assert colwise_rel(Wgot, Wref) <= tol
+        if case == "W_no_gram":
+            assert np.array_equal(Wgot, Wref) or colwise_rel(Wgot, Wref) <= tol
+    if wtw is not None:
+        G = wtw.get().astype(float)
+        assert np.all(np.abs(G - ol.gram(1.0, Wref, Wref, 0.0, np.zeros((nvec, nvec)))) <=
+                      _gram_bound(tol, Wref, Wref))
+        assert np.array_equal(G, G.T)  # the mirrored tiles are exact copies
+    if vtw is not None:
+        G = vtw.get().astype(float)
+        assert np.all(np.abs(G - ol.gram(1.0, Vf, Wref, 0.0, np.zeros((nvec, nvec)))) <= _gram_bound(tol, Vf, Wref))
+    for o in (v, w, wtw, vtw, A):
+        if o is not None:
+            o.delete()
+
+
+@pytest.mark.parametrize("nvec", [8, 32])
+def test_fused_spmv_mvTmv_large(be64, nvec):
+    """one-pass Gram epilogue over a persistent grid (300000 rows): both Grams against the oracle"""
+    n = 300000
+    A = be64.banded_random(n, 9, 100, 40.0, 31)
+    ro, ci, va = A.crs()
+    ref = ol.Crs(n, n, ro, ci, va)
+    rng = np.random.default_rng(nvec)
+    V = rng.uniform(-1, 1, (n, nvec))
+    m = be64.map(n)
+    v, w = be64.mvec(m, nvec, V), be64.mvec(m, nvec)
+    wtw, vtw = be64.sdmat(nvec, nvec), be64.sdmat(nvec, nvec)
+    be64.call("fused_spmv_mvTmv", be64.s(1.3), A.h, v.h, be64.s(0.0), w.h, wtw.h, vtw.h)
+    Wref = ol.spmv(ref, 1.3, V, 0.0, np.zeros((n, nvec)))
+    assert colwise_rel(w.get(), Wref) <= 1e-12
+    assert np.all(np.abs(wtw.get() - ol.gram(1.0, Wref, Wref, 0.0, np.zeros((nvec, nvec)))) <=
+                  _gram_bound(1e-12, Wref, Wref))
+    assert np.all(np.abs(vtw.get() - ol.gram(1.0, V, Wref, 0.0, np.zeros((nvec, nvec)))) <=
+                  _gram_bound(1e-12, V, Wref))
+    for o in (v, w, wtw, vtw, A):
+        o.delete()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [1, 4])
+def test_fused_spmv_pair(be64, be32, dtype, nvec):
+    """fused_spmv_pair: Y = alpha*(shifts1[j]*A1 + shifts2[j]*A2) X_j + beta*Y
+    (phist_kernels_fused_decl.h:45, kernels_no_fused_spmv_pair.cpp:9-49)"""
+    be = be64 if dtype == np.float64 else be32
+    A1, A2 = be.read_mm(mm("Dsprandn25")), be.read_mm(mm("Dspshift25"))
+    r1, r2 = ol.crs_from_mm(mm("Dsprandn25")), ol.crs_from_mm(mm("Dspshift25"))
+    rng = np.random.default_rng(nvec + 3)
+    X = rng.uniform(-1, 1, (25, nvec)).astype(dtype)
+    Y0 = rng.uniform(-1, 1, (25, nvec)).astype(dtype)
+    s1 = rng.uniform(-2, 2, nvec).astype(dtype)
+    s2 = rng.uniform(-2, 2, nvec).astype(dtype)
+    m = be.map(25)
+    x, y = be.mvec(m, nvec, X), be.mvec(m, nvec, Y0)
+    be.call("fused_spmv_pair", be.s(0.7), _p(s1), A1.h, _p(s2), A2.h, x.h, be.s(-0.5), y.h)
+    Xf = X.astype(float)
+    t1 = ol.spmv(r1, 0.7, Xf, 0.0, np.zeros((25, nvec)))
+    t2 = ol.spmv(r2, 0.7, Xf, 0.0, np.zeros((25, nvec)))
+    want = s1.astype(float) * t1 + s2.astype(float) * t2 - 0.5 * Y0.astype(float)
+    assert colwise_rel(y.get().astype(float), want) <= RTOL[dtype]
+    for o in (x, y, A1, A2):
+        o.delete()
+
+
+def test_mvec_QR_not_implemented(be64):
+    """builtin's mvec_QR returns -99 so that callers fall back to CholQR (kernels_def.hpp:939-947)"""
+    m = be64.map(100)
+    v = be64.mvec(m, 4, np.random.default_rng(0).uniform(-1, 1, (100, 4)))
+    r = be64.sdmat(4, 4)
+    assert be64.call("mvec_QR", v.h, r.h, allow=(0, -99)) == -99
+    for o in (v, r):
+        o.delete()
+    m.delete()

@@ -188,6 +188,37 @@ def test_oracle_inplace_vs_builtin(k, m):
     assert np.array_equal(ol.times_sdmat_inplace(V, M), Vr[:, :m])
 
 
+@needs_ref
+@pytest.mark.parametrize("k,m", [(4, 4), (8, 3), (5, 7), (16, 16)])
+@pytest.mark.parametrize("beta", [0.0, 0.5])
+def test_oracle_fused_mvsd_mvTmv_vs_builtin(k, m, beta):
+    """fused_mvsd_mvTmv (W = alpha*V*C + beta*W, D = W'W) as the oracle composes it from times_sdmat and
+    gram, against the flang-built dgemm_sB_augmented_generic (gemm_kernels_sB_augmented.F90:381-428),
+    one thread"""
+    rng = np.random.default_rng(13 + k)
+    n = 700
+    V = rng.uniform(-1, 1, (n, k))
+    C = np.asfortranarray(rng.uniform(-1, 1, (k, m)))
+    W = rng.uniform(-1, 1, (n, m))
+    I = ctypes.c_int
+    D = ctypes.c_double
+    Wr = W.copy()
+    Dr = np.zeros((m, m), order="F")
+    # dgemm_sB_augmented_generic(m=nrows, n=ncols W, k, alpha, A=V, lda, B=C(k,n), beta, C=W, ldc, D(n,n))
+    ref.dgemm_sb_augmented_generic_(ctypes.byref(I(n)), ctypes.byref(I(m)), ctypes.byref(I(k)), ctypes.byref(D(0.8)),
+                                    ol.p(V), ctypes.byref(I(k)), ol.p(C), ctypes.byref(D(beta)), ol.p(Wr),
+                                    ctypes.byref(I(m)), ol.p(Dr))
+    Wo = ol.times_sdmat(0.8, V, C, beta, W)
+    assert np.allclose(Wo, Wr, rtol=1e-13, atol=1e-13)
+    Do = ol.gram(1.0, Wo, Wo, 0.0, np.zeros((m, m)))
+    assert np.allclose(Do, Dr, rtol=1e-12, atol=1e-12 * n)
+    # the test catches a Gram of the old W or of V*C without beta*W
+    if beta != 0.0:
+        assert not np.allclose(ol.gram(1.0, W, W, 0.0, np.zeros((m, m))), Dr, rtol=1e-6)
+        W0 = ol.times_sdmat(0.8, V, C, 0.0, W)
+        assert not np.allclose(ol.gram(1.0, W0, W0, 0.0, np.zeros((m, m))), Dr, rtol=1e-6)
+
+
 def test_oracle_gram_long_double():
     rng = np.random.default_rng(9)
     n, k, m = 2000, 6, 4
