@@ -182,6 +182,18 @@ void phist_amd_DblockedMINRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, co
                                       phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[], int maxIter,
                                       int fp32, int nIter[], double relres[], int* iflag);
 
+/* Any number nsys of shifted systems (A + shifts[j] I) x_j = b_j (A symmetric; B and X hold nsys
+   columns) solved by blocked MINRES in a block of `slots` columns (1..64): when a system converges or
+   reaches maxIter its solution is stored in X(:,j) and its slot takes the next system while the
+   others keep iterating -- the replace-converged loop of
+   src/jada/phist_jadaCorrectionSolver_def.hpp:428-466 (there over the correction equations; here over
+   plain shifted solves).  Same stopping rule, precision switch and outputs as
+   phist_amd_DblockedMINRES_shifted; iflag > 0 = number of unconverged systems. */
+void phist_amd_DblockedMINRES_shifted_refill(phist_Dconst_sparseMat_ptr A, int nsys, const double shifts[],
+                                             phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[],
+                                             int maxIter, int fp32, int slots, int nIter[], double relres[],
+                                             int* iflag);
+
 /* Restarted GMRES(restart) on nsys <= 64 shifted systems (A + shifts[j] I) x_j = b_j of a general
    (nonsymmetric) matrix of this library, one Krylov basis per system as in
    src/krylov/phist_blockedgmres_def.hpp:453-1010 (restartedBlockedGMRES :1029-1200); X holds the initial

@@ -296,6 +296,22 @@ class Backend:
             raise PhistError("blockedMINRES_shifted", fl.value)
         return it, rr, fl.value
 
+    def minres_refill(self, A, shifts, B, X, tol, max_iter, slots, fp32=False):
+        """phist_amd_DblockedMINRES_shifted_refill: len(shifts) systems through `slots` columns, converged
+        columns replaced by the next system; returns (iters, relres, unconverged)"""
+        k = len(shifts)
+        sh = np.ascontiguousarray(shifts, np.float64)
+        tl = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, np.float64), (k,)))
+        it = np.zeros(k, np.int32)
+        rr = np.zeros(k)
+        fl = ctypes.c_int(0)
+        self.L.phist_amd_DblockedMINRES_shifted_refill(A.h, ctypes.c_int(k), _ptr(sh), B.h, X.h, _ptr(tl),
+                                                       ctypes.c_int(max_iter), ctypes.c_int(1 if fp32 else 0),
+                                                       ctypes.c_int(slots), _ptr(it), _ptr(rr), ctypes.byref(fl))
+        if fl.value < 0:
+            raise PhistError("blockedMINRES_shifted_refill", fl.value)
+        return it, rr, fl.value
+
     def set_minres_graphs(self, enable, iters_per_call=0, graph_len=4):
         """device MINRES: queue up to iters_per_call iterations per step; with enable, replay runs of
         graph_len iterations from captured HIP graphs"""

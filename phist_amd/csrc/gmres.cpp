@@ -24,7 +24,9 @@
 // maxIter.  At the end of a cycle x_j += V_j y_j (y from the triangular
 // solve); a restart recomputes the true residual b - (A + sI) x, and the
 // systems that finished leave: the next cycle runs on a block of the others
-// only (the basis shrinks with them).
+// only (the basis shrinks with them).  With a projection basis Q the
+// operator is P (A + s I), P = I - QQ' (the Jacobi-Davidson correction
+// equations; right-hand sides orthogonal to Q keep the iterates so).
 #include "solvers.hpp"
 #include <cmath>
 #include <limits>
@@ -33,9 +35,18 @@ namespace phx {
 
 template <typename T, typename TI>
 GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const Mvec<T>* B, Mvec<T>* X,
-                     const std::vector<double>& tol, int maxIter, int m) {
+                     const std::vector<double>& tol, int maxIter, int m, const Mvec<TI>* Q) {
   const int k = B->nvec;
   const Map* map = B->map;
+  if (Q && Q->nvec == 0) Q = nullptr;
+  // the projected operator P (A + s I) with P = I - QQ' (the correction equations of
+  // Jacobi-Davidson, phist_jadaOp_def.hpp:192-268): w -= Q (Q'w) after every product
+  auto project = [&](Mvec<TI>* w) {
+    if (!Q) return;
+    SdBox<TI> C(Q->nvec, w->nvec);
+    gemm_gram<TI>(TI(1), Q, w, TI(0), C.get(), false);
+    gemm_times_sd<TI>(TI(-1), Q, C.get(), TI(1), w);
+  };
   PHX_REQUIRE(m >= 1 && m <= 256, PHIST_INVALID_INPUT, "GMRES basis size %d (1..256)", m);
   PHX_REQUIRE((int64_t)(m + 1) * k <= 4096, PHIST_INVALID_INPUT, "GMRES basis (m+1)*k = %d columns too wide",
               (m + 1) * k);
@@ -68,6 +79,7 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
 
   auto residual = [&](std::vector<double>& nrm) {  // w = b - (A + sI) x on the compact blocks, norms
     op.apply_shifted(TI(-1), s.data(), xc->get(), TI(0), wc->get(), nullptr);
+    project(wc->get());
     std::vector<TI> one(kk, TI(1));
     blas_axpby<TI>(one.data(), true, bc->get(), TI(1), wc->get());
     std::vector<TI> d(kk);
@@ -176,6 +188,7 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
       View<TI> vi(Vb.get(), i * kk, (i + 1) * kk - 1);
       View<TI> Vi(Vb.get(), 0, (i + 1) * kk - 1);
       op.apply_shifted(TI(1), s.data(), vi.get(), TI(0), w.get(), nullptr);
+      project(w.get());
       std::vector<double> hc((size_t)(i + 1) * kk, 0.0);
       std::vector<TI> d(kk), inv(kk);
       if (cgs) {  // classical Gram-Schmidt twice in three sweeps, the norms in the last
@@ -315,9 +328,12 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
 
 template GmresOut gmres_solve<double, double>(const Operator<double>&, const std::vector<double>&,
                                               const Mvec<double>*, Mvec<double>*, const std::vector<double>&, int,
-                                              int);
+                                              int, const Mvec<double>*);
 template GmresOut gmres_solve<double, float>(const Operator<float>&, const std::vector<double>&, const Mvec<double>*,
-                                             Mvec<double>*, const std::vector<double>&, int, int);
+                                             Mvec<double>*, const std::vector<double>&, int, int,
+                                             const Mvec<float>*);
+template GmresOut gmres_solve<float, float>(const Operator<float>&, const std::vector<float>&, const Mvec<float>*,
+                                            Mvec<float>*, const std::vector<double>&, int, int, const Mvec<float>*);
 
 }  // namespace phx
 

@@ -478,6 +478,30 @@ class JadaRun {
       std::vector<double> inner_rr(k, 0.0);
       if (opts.innerSolvType == phist_NO_LINSOLV) {
         blas_copy<T>(rhs.get(), t.get());
+      } else if (opts.innerSolvType == phist_GMRES) {
+        // blocked GMRES on P (A - sigma I) P, P = I - QQ' (phist_jadaCorrectionSolver_def.hpp:499-512),
+        // from t = 0, basis size innerSolvMaxBas (innerSolvMaxIters when unset, :37-39): no restart
+        // within the iteration limit by default.  FP32 operator and basis on the FP32 copy of A.
+        const int m = std::max(1, std::min(opts.innerSolvMaxBas > 0 ? opts.innerSolvMaxBas : innerMaxIters, 256));
+        blas_put_value<T>(t.get(), T(0));
+        GmresOut r;
+        if (A32) {
+          for (double& x : tl) x = std::max(x, 1.0e-6);  // FP32 solves stagnate below this
+          const Mvec<float>* q32 = nullptr;
+          if (Qtil->nvec > 0) {
+            if (!q32_.p || q32_->nvec != Qtil->nvec) {
+              MvBox<float> fresh(Qout->map, Qtil->nvec);
+              std::swap(q32_.p, fresh.p);
+            }
+            copy_any<T, float>(Qtil.get(), q32_.get());
+            q32 = q32_.get();
+          }
+          r = gmres_solve<T, float>(*A32, s, rhs.get(), t.get(), tl, innerMaxIters, m, q32);
+        } else {
+          r = gmres_solve<T, T>(A, s, rhs.get(), t.get(), tl, innerMaxIters, m, Qtil.get());
+        }
+        inner_it = r.iters;
+        inner_rr = r.relres;
       } else if (A32) {
         for (double& x : tl) x = std::max(x, 1.0e-6);  // FP32 solves stagnate below this
         if (!solve32_[k]) solve32_[k].reset(new MinresSolver<T, float>(*A32, Qout->map, k));
@@ -609,6 +633,7 @@ class JadaRun {
   // correction-equation solvers by number of systems (kept between outer iterations)
   std::vector<std::unique_ptr<MinresSolver<T, T>>> solve_;
   std::vector<std::unique_ptr<MinresSolver<T, float>>> solve32_;
+  MvBox<float> q32_;  // FP32 copy of the projection basis for FP32 GMRES correction solves
   int nV = 0, nConv = 0, iter = 0;
   bool done_ = false;
   std::vector<double> theta, resNorm, resScaling, innerTol, lastOuterRes, hb, qh;
@@ -817,6 +842,43 @@ void phist_amd_DblockedMINRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, co
         relres[j] = r.relres[j];
       }
       unconv = r.unconverged;
+    }
+    *iflag = unconv;
+  });
+}
+
+void phist_amd_DblockedMINRES_shifted_refill(phist_Dconst_sparseMat_ptr A, int nsys, const double shifts[],
+                                             phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[],
+                                             int maxIter, int fp32, int slots, int nIter[], double relres[],
+                                             int* iflag) {
+  *iflag = 0;
+  guard(iflag, "phist_amd_DblockedMINRES_shifted_refill", [&] {
+    const auto* M = reinterpret_cast<const SparseMat<double>*>(A);
+    const auto* b = reinterpret_cast<const Mvec<double>*>(B);
+    auto* x = reinterpret_cast<Mvec<double>*>(X);
+    PHX_REQUIRE(M && b && x && shifts && tol && nIter && relres, PHIST_INVALID_INPUT, "null argument");
+    PHX_REQUIRE(nsys >= 1 && b->nvec == nsys && x->nvec == nsys, PHIST_INVALID_INPUT,
+                "nsys %d must match B and X", nsys);
+    PHX_REQUIRE(slots >= 1 && slots <= 64, PHIST_INVALID_INPUT, "slots %d not in 1..64", slots);
+    PHX_REQUIRE(M->nrows == M->ncols && b->nrows == M->nrows && x->nrows == M->nrows, PHIST_INVALID_INPUT,
+                "shapes do not match the square matrix");
+    std::vector<double> s(shifts, shifts + nsys), tl(tol, tol + nsys);
+    int unconv = 0;
+    auto put = [&](const auto& r) {
+      for (int j = 0; j < nsys; j++) {
+        nIter[j] = r.iters[j];
+        relres[j] = r.relres[j];
+      }
+      unconv = r.unconverged;
+    };
+    if (fp32) {
+      Operator<float> op;
+      op.A = sparse_fp32_twin(M);
+      put(minres_refill_solve<double, float>(op, s, b, x, tl, maxIter, std::min(slots, nsys)));
+    } else {
+      Operator<double> op;
+      op.A = M;
+      put(minres_refill_solve<double, double>(op, s, b, x, tl, maxIter, std::min(slots, nsys)));
     }
     *iflag = unconv;
   });

@@ -793,6 +793,47 @@ MinresOut<TI> minres_solve(const Operator<TI>& op, const std::vector<T>& s, cons
   return solver.solve(s, Q, B, X, tol, maxIter);
 }
 
+// nsys systems (B, X with nsys columns) through a batch of `slots` columns:
+// a system whose slot stops (converged, broken down or at maxIter) has its
+// solution stored and the slot is loaded with the next system while the
+// others keep iterating (phist_jadaCorrectionSolver_def.hpp:428-466)
+template <typename T, typename TI>
+MinresOut<TI> minres_refill_solve(const Operator<TI>& op, const std::vector<T>& s, const Mvec<T>* B, Mvec<T>* X,
+                                  const std::vector<double>& tol, int maxIter, int slots) {
+  const int nsys = B->nvec;
+  MinresBatch<TI> mb(op, B->map, slots);
+  std::vector<int> owner(slots, -1);
+  MinresOut<TI> out;
+  out.iters.assign(nsys, 0);
+  out.relres.assign(nsys, 0.0);
+  int next = 0, done = 0;
+  auto fill = [&] {
+    int j;
+    while (next < nsys && (j = mb.free_slot()) >= 0) {
+      mb.load(j, B, next, (TI)s[next], tol[next], maxIter);
+      owner[j] = next++;
+    }
+  };
+  fill();
+  while (done < nsys) {
+    mb.iterate();
+    for (int j = 0; j < slots; j++) {
+      if (owner[j] < 0 || mb.active(j)) continue;
+      const int c = owner[j];
+      mb.store(j, X, c);
+      out.iters[c] = mb.iters(j);
+      out.relres[c] = mb.relres(j);
+      if (!mb.converged(j)) out.unconverged++;
+      mb.release(j);
+      owner[j] = -1;
+      done++;
+    }
+    fill();
+  }
+  sync_stream();
+  return out;
+}
+
 // restarted GMRES(m) on k shifted systems (gmres.cpp), x from X as the
 // initial guess; iters = Arnoldi steps per system, relres = ||r|| / ||r0||
 struct GmresOut {
@@ -802,7 +843,7 @@ struct GmresOut {
 };
 template <typename T, typename TI>
 GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const Mvec<T>* B, Mvec<T>* X,
-                     const std::vector<double>& tol, int maxIter, int m);
+                     const std::vector<double>& tol, int maxIter, int m, const Mvec<TI>* Q = nullptr);
 
 // resumable block Jacobi-Davidson on a sparse matrix of this library
 // (defined in jada.cpp): start builds the start-up basis, step runs one outer

@@ -6,7 +6,10 @@ Laplacian against the closed form sum_d (2 - 2 cos(pi i_d / (n+1))); spin
 chain eigenvalues against dense diagonalisation of the oracle's matrix; the
 returned partial Schur form by ||A Q - Q R|| and Q'Q = I (the invariants
 test/jada/TestSubspaceJaDa_def.hpp checks).  The FP32-inner (mixed) runs must
-reach the same FP64 tolerance.
+reach the same FP64 tolerance.  The eigensolver tests run with both
+correction-equation solvers the reference offers (innerSolvType MINRES and
+GMRES, phist_jadaCorrectionSolver_def.hpp:499-512; GMRES is the reference's
+default, phist_jadaOpts.cpp:40).
 """
 import itertools
 
@@ -103,9 +106,13 @@ def check_schur(be, A, Q, R, nconv, tol):
     assert np.allclose(Qh.T @ Qh, np.eye(nconv), atol=1e-10)
 
 
+INNER = ["MINRES", "GMRES"]
+
+
+@pytest.mark.parametrize("inner", INNER)
 @pytest.mark.parametrize("fp32", [False, True])
 @pytest.mark.parametrize("bs", [1, 2, 4])
-def test_jada_laplace_smallest(be64, bs, fp32):
+def test_jada_laplace_smallest(be64, bs, fp32, inner):
     """box of incommensurate sides: the smallest eigenvalues are simple (a
     Krylov space grown from one start vector holds one copy of a multiple
     eigenvalue in exact arithmetic, so a cube is not a fair test)"""
@@ -115,7 +122,7 @@ def test_jada_laplace_smallest(be64, bs, fp32):
     tol = 1e-8
     ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, "SR", tol, block_size=bs, min_bas=12 + bs,
                                                     max_bas=24 + 2 * bs, max_iters=400, inner_max_iters=20,
-                                                    inner_fp32=fp32)
+                                                    inner_fp32=fp32, inner=inner)
     assert nconv >= k, (nconv, niter, res)
     want = laplace_eigs(nx, ny, nz, k)
     assert np.allclose(np.sort(ev[:k]), want, rtol=0, atol=1e-7), (np.sort(ev[:k]), want)
@@ -125,15 +132,16 @@ def test_jada_laplace_smallest(be64, bs, fp32):
         o.delete()
 
 
+@pytest.mark.parametrize("inner", INNER)
 @pytest.mark.parametrize("which", ["SR", "LR", "LM"])
-def test_jada_spin_chain(be64, which):
+def test_jada_spin_chain(be64, which, inner):
     L = 10
     A = be64.spinchain_sz(L, L // 2)
     dense = ol.rows_to_dense(ol.spin_sz_rows(L, L // 2, 0), A.nrows)
     w = np.linalg.eigvalsh(dense)
     k = 4
     ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, which, 1e-8, block_size=2, min_bas=12, max_bas=24,
-                                                    max_iters=400)
+                                                    max_iters=400, inner=inner)
     assert nconv >= k, (nconv, niter, res)
     check_found(ev[:k], w, which)
     check_schur(be64, A, Q, R, k, 1e-8)
@@ -141,8 +149,9 @@ def test_jada_spin_chain(be64, which):
         o.delete()
 
 
+@pytest.mark.parametrize("inner", INNER)
 @pytest.mark.parametrize("fp32", [False, True])
-def test_jada_spin_chain_L14(be64, fp32):
+def test_jada_spin_chain_L14(be64, fp32, inner):
     """the reference's integration case (drivers/integration_tests.cmake:9-10:
     subspacejada on spinSZ14 to 1e-8), smallest eigenvalues vs dense"""
     L = 14
@@ -151,7 +160,8 @@ def test_jada_spin_chain_L14(be64, fp32):
     w = np.linalg.eigvalsh(dense)
     k = 4
     ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, "SR", 1e-8, block_size=4, min_bas=16, max_bas=32,
-                                                    max_iters=200, inner_max_iters=10, inner_fp32=fp32)
+                                                    max_iters=200, inner_max_iters=10, inner_fp32=fp32,
+                                                    inner=inner)
     assert nconv >= k, (nconv, niter, res)
     check_found(ev[:k], w, "SR")
     check_schur(be64, A, Q, R, k, 1e-8)
@@ -239,5 +249,22 @@ def test_jada_invariant_start_space(be64, tmp_path):
     Qh = Q.get()[:, :2]
     assert np.allclose(Qh.T @ Qh, np.eye(2), atol=1e-10)
     assert np.allclose(d[:, None] * Qh, Qh * 1.0, atol=1e-7)
+    for o in (Q, R, A):
+        o.delete()
+
+
+@pytest.mark.parametrize("inner", INNER)
+@pytest.mark.parametrize("fp32", [False, True])
+def test_jada_laplace_config1_shape(be64, fp32, inner):
+    """the CPU-reference JD configuration's solver settings on a smaller box: 8 smallest, blockSize 4,
+    minBas 16, maxBas 32, at most 10 inner iterations, tol 1e-8 (incommensurate sides, simple spectrum)"""
+    nx, ny, nz = 17, 19, 23
+    A = be64.laplace3d(nx, ny, nz)
+    ev, res, nconv, niter, Q, R = be64.subspacejada(A, 8, "SR", 1e-8, block_size=4, min_bas=16, max_bas=32,
+                                                    max_iters=300, inner_max_iters=10, inner_fp32=fp32, inner=inner)
+    print(f"lap {nx}x{ny}x{nz} inner {inner} fp32 {fp32}: {nconv} converged in {niter} iterations")
+    assert nconv >= 8, (nconv, niter, res)
+    assert np.allclose(np.sort(ev[:8]), laplace_eigs(nx, ny, nz, 8), rtol=0, atol=1e-7)
+    check_schur(be64, A, Q, R, 8, 1e-8)
     for o in (Q, R, A):
         o.delete()
