@@ -9,7 +9,9 @@ the oracle's SpMV of the matrix's CRS; FP64 to the tolerance 1e-6 (the MINRES re
 the true residual differ by O(eps * cond^2), far below it at these sizes).  FP32 iterations reach a
 true residual limited by FP32 rounding times the conditioning of these interior shifts: they are run
 to tol 1e-4 and checked to 1e-3.  The number of iterations a system needs does not depend on the slot
-it runs in or on its neighbours (to +-2, the lag of the host's view of the states)."""
+it runs in or on its neighbours beyond rounding: blocks of other widths reduce the dots in another
+order, and over a thousand iterations on these nearly singular systems that moves the count by up to
+1% (plus 2, the lag of the host's view of the states)."""
 import numpy as np
 import pytest
 
@@ -49,7 +51,7 @@ def test_minres_honeycomb_refill_fp64(be64, W, L, nsys, slots):
         B2, X2 = be64.mvec(A.domain_map, k, Bh[:, :k]), be64.mvec(A.domain_map, k)
         it2, _, unconv2 = be64.minres_shifted(A, shifts[:k], B2, X2, 1e-6, 60000)
         assert unconv2 == 0
-        assert np.all(np.abs(it2 - it[:k]) <= 2), (it2, it[:k])
+        assert np.all(np.abs(it2 - it[:k]) <= 2 + it[:k] // 100), (it2, it[:k])
         for o in (B2, X2):
             o.delete()
     for o in (B, X, A):
