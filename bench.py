@@ -634,6 +634,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the library's STREAM copy kernel (1 GiB in, 1 GiB out per launch) so a "
                          "counter profile of this command holds a known-size reference dispatch")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run the library's known-size gather kernel (every BYTES-byte segment of 1 GiB "
+                         "read once, scrambled order) for a counter profile of gathers of that size")
+    ap.add_argument("--halfband", type=int, default=50000,
+                    help="--op spmv: columns drawn within +-HALFBAND of the diagonal (50000 is the kernel-bench "
+                         "matrix; other values are a band sweep, not the headline line)")
+    ap.add_argument("--lane-bytes", type=int, default=0,
+                    help="--op spmv: bytes of x each SpMV lane gathers per nonzero (0 = library default)")
+    ap.add_argument("--blocks-per-cu", type=int, default=-1,
+                    help="--op spmv: SpMV workgroups per CU (-1 = library table)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -650,6 +660,14 @@ def main():
         L.phist_bench_stream_copy(ctypes.byref(mean_bw), ctypes.byref(max_bw), ctypes.byref(iflag))
         print(f"bench: STREAM copy {mean_bw.value:.0f} GB/s mean, {max_bw.value:.0f} GB/s max",
               file=sys.stderr, flush=True)
+    if args.gather_calibration:
+        mean_bw, max_bw = ctypes.c_double(), ctypes.c_double()
+        L.phist_amd_bench_gather(args.gather_calibration, ctypes.byref(mean_bw), ctypes.byref(max_bw),
+                                 ctypes.byref(iflag))
+        if iflag.value:
+            raise RuntimeError(f"phist_amd_bench_gather iflag={iflag.value}")
+        print(f"bench: gather {args.gather_calibration}-byte segments once over 1 GiB: {mean_bw.value:.0f} GB/s "
+              f"mean, {max_bw.value:.0f} GB/s max", file=sys.stderr, flush=True)
     if args.op == "jada":
         return jada_bench(be, args)
     if args.op == "minres":
@@ -663,9 +681,13 @@ def main():
     if args.op != "spmv":
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
+    L.phist_amd_set_spmv_lane_bytes(args.lane_bytes, ctypes.byref(iflag))
+    L.phist_amd_set_spmv_blocks_per_cu(args.blocks_per_cu, ctypes.byref(iflag))
+    if iflag.value:
+        raise RuntimeError(f"SpMV launch settings rejected (iflag={iflag.value})")
     N, nvec = args.rows, args.nvec
     t0 = time.perf_counter()
-    A = be.banded_random(N, 32, 50000, 40.0, 2024)
+    A = be.banded_random(N, 32, args.halfband, 40.0, 2024)
     t_build = time.perf_counter() - t0
     m = be.map(N)
     x = be.mvec(m, nvec)
@@ -703,7 +725,9 @@ def main():
     k_avg_ms = kms.value / max(1, kcount.value)
     value = nbytes / (ms_step * 1e-3) / 1e9
     achieved = nbytes / (k_avg_ms * 1e-3) / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles", f"sell_spmv_fused_nvec{nvec}_traffic.txt"))
+    default_run = args.halfband == 50000 and N == 4194304 and args.lane_bytes == 0 and args.blocks_per_cu < 0
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"sell_spmv_fused_nvec{nvec}_traffic.txt")) \
+        if default_run else None
 
     cpu = None
     if not args.no_cpu:
@@ -724,12 +748,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic (banded random matrix from a seeded generator, KISS random block vector)",
         "config": {
-            "workload": "fused_spmv_mvdot, banded random 4M rows x 32 nnz/row (+-50k band, diag 40), "
-                        "SELL-C-sigma C=32 sigma=256, FP64, alpha=1 beta=0, both dots",
+            "workload": f"fused_spmv_mvdot, banded random {N} rows x 32 nnz/row (+-{args.halfband} band, diag 40), "
+                        f"SELL-C-sigma C=32 sigma=256, FP64, nvec={nvec}, alpha=1 beta=0, both dots",
             "rows": N, "nnz": A.nnz, "nvec": nvec, "bytes_per_step": nbytes,
             "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 1),
             "kernel_avg_ms": round(k_avg_ms, 4), "kernel_launches_timed": kcount.value,
             "parallelism": "single GPU",
+            **({} if args.lane_bytes == 0 else {"lane_bytes": args.lane_bytes}),
+            **({} if args.blocks_per_cu < 0 else {"blocks_per_cu": args.blocks_per_cu}),
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

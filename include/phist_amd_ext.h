@@ -28,6 +28,13 @@ void phist_amd_set_grid_blocks(int nblocks, int* iflag);
  * blocks of >= 4 columns, whose gathers then keep a wide band's x rows
  * L2-resident, 3 for such rows and narrower blocks, 6 for shorter rows) */
 void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
+/* bytes of a block row each lane of the SpMV gathers per nonzero (0 = the
+ * default, one 16-byte vector; sizeof(scalar) = one scalar per lane; 32 or 64
+ * = two or four 16-byte vectors per lane, taken when the block width and
+ * leading dimension are multiples of that many scalars).  Every choice gives
+ * the same result bit for bit: a row's sum is always formed by the lane that
+ * owns its columns, in the row's entry order. */
+void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag);
 /* the device-resident MINRES iteration (blocked MINRES, the eigensolver's
  * correction equations, the service loop): iters_per_call iterations are
  * queued per host step, never more than an unfinished system may still take
@@ -53,6 +60,11 @@ void phist_amd_device_info(char* name, int name_len, int* num_cu, int* iflag);
  * number of timed launches since the last enable */
 void phist_amd_kernel_timer(int enable, int* iflag);
 void phist_amd_kernel_timer_read(double* total_ms, int64_t* count, int* iflag);
+/* known-size gather: every seg_bytes-byte segment (16..256, a power of two)
+ * of a 1 GiB device array read exactly once in a scrambled order, one 16-byte
+ * load per lane; mean and best GB/s of 20 launches.  A counter profile of it
+ * calibrates the L2 fetch counters for gathers of that size. */
+void phist_amd_bench_gather(int seg_bytes, double* mean_bw, double* max_bw, int* iflag);
 
 /* the global KISS stream: reset to the initial state, jump ahead by n draws,
  * draw n doubles in [-1,1) */

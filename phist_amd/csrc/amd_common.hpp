@@ -95,6 +95,8 @@ struct Runtime {
   // workgroups per CU for the SpMV: the x gathers of a +-band thrash L2 when
   // too many rows are in flight, two workgroups per CU measured best
   int spmv_blocks_per_cu = -1;  // < 0: spmv_table_blocks_per_cu
+  // bytes of a block row each SpMV lane gathers per nonzero (0: one 16-byte vector)
+  int spmv_lane_bytes = 0;
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
   // ... when the wider side has at least gram_mfma_wide columns, or an

@@ -30,8 +30,10 @@ __device__ inline TileWalk xcd_tiles(int64_t ntiles) {
   return w;
 }
 
+// (at most 16-byte alignment: the widest load is 16 bytes, so wider blocks
+// are several loads and need only that)
 template <typename T, int N>
-struct alignas(sizeof(T) * N) VecN {
+struct alignas(sizeof(T) * N < 16 ? sizeof(T) * N : 16) VecN {
   T v[N];
 };
 

@@ -49,6 +49,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SELL_SIGMA")) r.sell_sigma = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRID_BLOCKS")) r.grid_blocks = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_SPMV_LANE_BYTES")) r.spmv_lane_bytes = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA_MIN_M")) r.mix_mfma_min_m = std::max(1, atoi(s));
@@ -297,6 +298,68 @@ __global__ void k_stream_triad(const double* __restrict__ a, const double* __res
     c[i] = a[i] + s * b[i];
 }
 
+// Known-size gather: every seg-byte segment of an n-double array is read
+// exactly once, LPR = seg/16 lanes per segment (one 16-byte load each), in
+// the scrambled order s(i) = (i * P) mod nseg (P odd, nseg a power of two: a
+// bijection), so consecutive groups of a wave touch unrelated cache lines
+// as the SpMV's x gathers do, and the bytes requested are exactly 8n.
+template <int LPR>
+__global__ void __launch_bounds__(256) k_gather_once(const double* __restrict__ a, int64_t nseg, double* out) {
+  const int q = threadIdx.x % LPR;
+  const int64_t g0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / LPR;
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x / LPR;
+  const uint64_t P = 0x9E3779B97F4A7C15ull;
+  double s0 = 0, s1 = 0;
+  for (int64_t g = g0; g < nseg; g += gs) {
+    const int64_t seg = (int64_t)(((uint64_t)g * P) & (uint64_t)(nseg - 1));
+    const double2 v = *reinterpret_cast<const double2*>(a + seg * (2 * LPR) + 2 * q);
+    s0 += v.x;
+    s1 += v.y;
+  }
+  if (s0 + s1 == 12345.678) out[0] = s0;  // keeps the loads alive
+}
+
+// one launch of k_gather_once over 1 GiB with seg_bytes-byte segments
+// (16..256), timed over reps after two untimed launches
+void run_gather(int seg_bytes, double* mean_bw, double* max_bw) {
+  ensure_init();
+  Runtime& r = rt();
+  PHX_REQUIRE(seg_bytes >= 16 && seg_bytes <= 256 && (seg_bytes & (seg_bytes - 1)) == 0, PHIST_INVALID_INPUT,
+              "segment size %d: a power of two in 16..256", seg_bytes);
+  const int64_t n = int64_t(1) << 27;  // 1 GiB
+  const int64_t nseg = n * 8 / seg_bytes;
+  double *a = nullptr, *c = nullptr;
+  PHX_HIP(hipMalloc(&a, n * 8));
+  PHX_HIP(hipMalloc(&c, 64));
+  PHX_HIP(hipMemsetAsync(a, 0, n * 8, r.stream));
+  const int blocks = r.num_cu * 8;
+  const int reps = 20;
+  double sum = 0, best = 0;
+  for (int it = -2; it < reps; it++) {
+    PHX_HIP(hipEventRecord(r.ev_start, r.stream));
+    switch (seg_bytes) {
+      case 16: hipLaunchKernelGGL(k_gather_once<1>, dim3(blocks), dim3(256), 0, r.stream, a, nseg, c); break;
+      case 32: hipLaunchKernelGGL(k_gather_once<2>, dim3(blocks), dim3(256), 0, r.stream, a, nseg, c); break;
+      case 64: hipLaunchKernelGGL(k_gather_once<4>, dim3(blocks), dim3(256), 0, r.stream, a, nseg, c); break;
+      case 128: hipLaunchKernelGGL(k_gather_once<8>, dim3(blocks), dim3(256), 0, r.stream, a, nseg, c); break;
+      default: hipLaunchKernelGGL(k_gather_once<16>, dim3(blocks), dim3(256), 0, r.stream, a, nseg, c); break;
+    }
+    PHX_HIP(hipGetLastError());
+    PHX_HIP(hipEventRecord(r.ev_stop, r.stream));
+    PHX_HIP(hipEventSynchronize(r.ev_stop));
+    float ms = 0;
+    PHX_HIP(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
+    if (it < 0) continue;
+    const double gbs = 8.0 * n / (ms * 1e-3) / 1e9;
+    sum += gbs;
+    best = std::max(best, gbs);
+  }
+  PHX_HIP(hipFree(a));
+  PHX_HIP(hipFree(c));
+  *mean_bw = sum / reps;
+  *max_bw = best;
+}
+
 // kind: 0 load, 1 store, 2 copy, 3 triad; returns mean/max GB/s over reps
 void run_stream(int kind, double* mean_bw, double* max_bw) {
   ensure_init();
@@ -505,6 +568,9 @@ void phist_bench_stream_copy(double* mean_bw, double* max_bw, int* iflag) {
 void phist_bench_stream_triad(double* mean_bw, double* max_bw, int* iflag) {
   guard(iflag, __func__, [&] { run_stream(3, mean_bw, max_bw); *iflag = 0; });
 }
+void phist_amd_bench_gather(int seg_bytes, double* mean_bw, double* max_bw, int* iflag) {
+  guard(iflag, __func__, [&] { run_gather(seg_bytes, mean_bw, max_bw); *iflag = 0; });
+}
 
 // ---- extension hooks (phist_amd_ext.h) ----
 void phist_amd_sparseMat_set_format(int C, int sigma, int* iflag) {
@@ -527,6 +593,15 @@ void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag) {
   guard(iflag, __func__, [&] {
     ensure_init();
     rt().spmv_blocks_per_cu = n < 0 ? -1 : n;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_REQUIRE(bytes == 0 || bytes == 4 || bytes == 8 || bytes == 16 || bytes == 32 || bytes == 64,
+                PHIST_INVALID_INPUT, "lane width %d bytes: 0, 4, 8, 16, 32 or 64", bytes);
+    rt().spmv_lane_bytes = bytes;
     *iflag = 0;
   });
 }

@@ -465,7 +465,19 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   bool vec_ok = (nvec % V16 == 0) && (x->lda % V16 == 0) && aligned16(x->dev());
   if (y) vec_ok = vec_ok && (y->lda % V16 == 0) && aligned16(y->dev());
   if (req.axpy_u) vec_ok = vec_ok && (U->lda % V16 == 0) && aligned16(U->dev());
-  const int VPL = vec_ok ? V16 : 1;
+  int VPL = vec_ok ? V16 : 1;
+  if (!gram) {  // the gathered bytes per lane and nonzero (same sums whichever is taken)
+    const int lb = rt().spmv_lane_bytes;
+    const int w = lb / (int)sizeof(T);
+    if (lb > 0 && lb <= (int)sizeof(T)) {
+      VPL = 1;
+    } else if (vec_ok && (w == 2 * V16 || w == 4 * V16)) {
+      bool ok = nvec % w == 0 && x->lda % w == 0;
+      if (y) ok = ok && y->lda % w == 0;
+      if (req.axpy_u) ok = ok && U->lda % w == 0;
+      if (ok) VPL = w;
+    }
+  }
   const int maxcols = 64 * VPL;
 
   SpmvArgs<T> a;
@@ -523,7 +535,10 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
       }
       continue;
     }
-    const int grid = VPL == 1 ? launch_xsc<T, 1>(a, lpr) : launch_xsc<T, V16>(a, lpr);
+    const int grid = VPL == 1     ? launch_xsc<T, 1>(a, lpr)
+                     : VPL == V16 ? launch_xsc<T, V16>(a, lpr)
+                     : VPL == 2 * V16 ? launch_xsc<T, 2 * V16>(a, lpr)
+                                      : launch_xsc<T, 4 * V16>(a, lpr);
     if (dots && req.dots_dev) {
       PHX_REQUIRE(cb == 0 && nc == nvec, PHIST_INVALID_INPUT, "device dots need one column block (%d columns)", nvec);
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,

@@ -452,3 +452,47 @@ def test_mvec_QR_not_implemented(be64):
     for o in (v, r):
         o.delete()
     m.delete()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [8, 16, 32])
+def test_spmv_lane_widths_bitexact(be64, be32, dtype, nvec):
+    """every gather width per lane (one scalar, one to four 16-byte vectors)
+    gives the default's result bit for bit, shifts, beta and dots included
+    (a row's sum is formed by the lane owning its columns, in entry order),
+    and the default agrees with the oracle"""
+    import ctypes
+    be = be64 if dtype == np.float64 else be32
+    n = 40000
+    A = be.banded_random(n, 17, 3000, 40.0, 99)
+    ro, ci, va = A.crs()
+    ref = ol.Crs(n, n, ro, ci, va.astype(float))
+    rng = np.random.default_rng(nvec + 5)
+    X = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    Y0 = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    shifts = rng.uniform(-2, 2, nvec).astype(dtype)
+    m = be.map(n)
+    x = be.mvec(m, nvec, X)
+    fl = ctypes.c_int(0)
+    out = {}
+    try:
+        for lb in (0, np.dtype(dtype).itemsize, 16, 32, 64):
+            be.L.phist_amd_set_spmv_lane_bytes(lb, ctypes.byref(fl))
+            assert fl.value == 0
+            y = be.mvec(m, nvec, Y0)
+            be.spmv_shifted(0.7, A, shifts, x, 0.3, y)
+            w = be.mvec(m, nvec, Y0)
+            ww, vw = be.fused_spmv_mvdot(1.1, A, x, 0.5, w)
+            out[lb] = (y.get(), w.get(), ww, vw)
+            y.delete()
+            w.delete()
+    finally:
+        be.L.phist_amd_set_spmv_lane_bytes(0, ctypes.byref(fl))
+    want = ol.spmv(ref, 0.7, X.astype(float), 0.3, Y0.astype(float), shifts=shifts.astype(float))
+    assert colwise_rel(out[0][0].astype(float), want) <= RTOL[dtype]
+    for lb, got in out.items():
+        for g, d in zip(got, out[0]):
+            assert np.array_equal(g, d), f"lane width {lb} bytes differs from the default"
+    for o in (x, A):
+        o.delete()
+    m.delete()
