@@ -27,6 +27,11 @@ import os
 import sys
 import time
 
+# CPU baseline threads are pinned one per physical core, as the builtin path
+# is run (set before any OpenMP runtime starts; a caller's own settings win)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -50,6 +55,36 @@ def read_traffic(path, key="traffic_bytes_per_launch"):
         if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def host_cpu():
+    """(CPU model, physical cores) of this host from /proc/cpuinfo"""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return model, (len(cores) or os.cpu_count() or 1)
+
+
+def cpu_info_fields(threads):
+    model, phys = host_cpu()
+    return {"cpu_model": model, "physical_cores": phys, "threads": threads,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES")}
 
 
 def cpu_baseline(A, n_sample, nvec, budget_s=12.0):
@@ -106,7 +141,7 @@ def cpu_baseline(A, n_sample, nvec, budget_s=12.0):
     sample = (f"first {n_sample} rows ({nnz_s} nnz) of the same matrix, nvec={nvec}, "
               f"spmv (alpha=1, beta=0) + 2 column dots, median of {len(times)} runs")
     return {"value": round(gbs, 2), "unit": "GB/s", "cores": threads, "kind": kind, "sample": sample,
-            "ms_per_step": round(t * 1e3, 3)}
+            "ms_per_step": round(t * 1e3, 3), **cpu_info_fields(threads)}
 
 
 def tall_skinny(be, args):
@@ -245,6 +280,47 @@ def jada_bench(be, args):
            "fp64": results["fp64"], "mixed": results["mixed"], "speedup_mixed_vs_fp64": round(speed, 3)}
     print(json.dumps(out), flush=True)
     A.delete()
+
+
+def tune_bench(be, args):
+    """The SpMV autotuner on the matrices of the workloads: per matrix and block width the kernel time per
+    product with the library's untuned choice (default SELL layout, launch table) and with the settings
+    the tuner picks over SELL layouts (C 16/32/64, sigma 1/256), gather width per lane and workgroups
+    per CU; one JSON line per matrix."""
+    L = be.L
+    pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    cases = [
+        ("banded 4194304 rows x 32 nnz (+-50k), FP64", lambda: be.banded_random(4194304, 32, 50000, 40.0, 2024),
+         [1, 4, 8, 16, 32]),
+        ("XXZ chain L=24 S_z=0, FP64", lambda: be.spinchain_sz(24, 12), [4, 16]),
+        ("honeycomb 2000x4000 sites (8M rows), FP64", lambda: be.honeycomb(2000, 4000, 0, 0, 0, 1.0, 77), [32]),
+        ("3D Laplacian 128^3, FP64", lambda: be.laplace3d(128, 128, 128), [1, 4, 16]),
+    ]
+    iflag = ctypes.c_int(0)
+    L.phist_amd_sparseMat_set_format(0, 0, ctypes.byref(iflag))
+    for name, make, widths in cases:
+        t0 = time.perf_counter()
+        A = make()
+        t_build = time.perf_counter() - t0
+        n = len(widths)
+        nv, dms, tms, st = np.array(widths, np.int32), np.zeros(n), np.zeros(n), np.zeros(4 * n, np.int32)
+        t0 = time.perf_counter()
+        L.phist_Damd_sparseMat_tune(A.h, n, pp(nv), 1 | 2, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
+        if iflag.value:
+            raise RuntimeError(f"tune {name}: iflag {iflag.value}")
+        t_tune = time.perf_counter() - t0
+        rows = []
+        for i, w in enumerate(widths):
+            C, sg, lb, bpc = (int(v) for v in st[4 * i:4 * i + 4])
+            gb = spmv_bytes(A.nnz, A.nrows, w) / 1e9
+            rows.append({"nvec": w, "untuned_ms": round(float(dms[i]), 4), "tuned_ms": round(float(tms[i]), 4),
+                         "gain": round(float(dms[i] / tms[i]), 3), "C": C, "sigma": sg, "lane_bytes": lb,
+                         "blocks_per_cu": bpc, "tuned_GBs": round(gb / (tms[i] * 1e-3), 1)})
+        out = {"metric": "SpMV autotuner: kernel ms per product, untuned vs tuned", "workload": name,
+               "rows": A.nrows, "nnz": A.nnz, "build_s": round(t_build, 2), "tune_s": round(t_tune, 2),
+               "widths": rows}
+        print(json.dumps(out), flush=True)
+        A.delete()
 
 
 def unit_random_block(be, m, k):
@@ -617,7 +693,8 @@ def main():
     ap.add_argument("--nvec", type=int, default=8)
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada", "minres", "serve", "carp", "gmres"],
+    ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada", "minres", "serve", "carp", "gmres",
+                                     "tune"],
                     default="spmv",
                     help="spmv (default): the fused SpMV line; gram / cholqr / mvsd (fused_mvsd_mvTmv, "
                          "W = V*C and W'W, beta=0): the tall-skinny configuration (16,777,216 x k, U(-1,1)) "
@@ -644,6 +721,8 @@ def main():
                     help="--op spmv: bytes of x each SpMV lane gathers per nonzero (0 = library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=-1,
                     help="--op spmv: SpMV workgroups per CU (-1 = library table)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="--op spmv: skip the autotuner (launch settings from the library's table)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -678,6 +757,8 @@ def main():
         return carp_bench(be, args)
     if args.op == "gmres":
         return gmres_bench(be, args)
+    if args.op == "tune":
+        return tune_bench(be, args)
     if args.op != "spmv":
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))
@@ -695,6 +776,19 @@ def main():
     y = be.mvec(m, nvec)
     nbytes = spmv_bytes(A.nnz, N, nvec)
     print(f"bench: matrix {N} rows, {A.nnz} nnz built in {t_build:.1f} s; nvec={nvec}", file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_tune and args.lane_bytes == 0 and args.blocks_per_cu < 0:
+        # the library's autotuner picks the launch settings for this width; the
+        # layout stays the configuration's SELL-32-256
+        nv, dms, tms, st = np.array([nvec], np.int32), np.zeros(1), np.zeros(1), np.zeros(4, np.int32)
+        pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        t0 = time.perf_counter()
+        L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 0, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
+        if iflag.value:
+            raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
+        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "untuned_kernel_ms": round(float(dms[0]), 4),
+                "tuned_kernel_ms": round(float(tms[0]), 4), "tune_s": round(time.perf_counter() - t0, 2)}
+        print(f"bench: autotuner {tune}", file=sys.stderr, flush=True)
 
     ww = np.zeros(nvec)
     vw = np.zeros(nvec)
@@ -725,7 +819,8 @@ def main():
     k_avg_ms = kms.value / max(1, kcount.value)
     value = nbytes / (ms_step * 1e-3) / 1e9
     achieved = nbytes / (k_avg_ms * 1e-3) / 1e9
-    default_run = args.halfband == 50000 and N == 4194304 and args.lane_bytes == 0 and args.blocks_per_cu < 0
+    default_run = args.halfband == 50000 and N == 4194304 and args.lane_bytes == 0 and args.blocks_per_cu < 0 \
+        and not args.no_tune
     traffic = read_traffic(os.path.join(ROOT, "profiles", f"sell_spmv_fused_nvec{nvec}_traffic.txt")) \
         if default_run else None
 
@@ -756,6 +851,7 @@ def main():
             "parallelism": "single GPU",
             **({} if args.lane_bytes == 0 else {"lane_bytes": args.lane_bytes}),
             **({} if args.blocks_per_cu < 0 else {"blocks_per_cu": args.blocks_per_cu}),
+            **({} if tune is None else {"autotuned": tune}),
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -35,6 +35,28 @@ void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
  * the same result bit for bit: a row's sum is always formed by the lane that
  * owns its columns, in the row's entry order. */
 void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag);
+/* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16
+ * widths) it times the product y = A*x on the device over the gather width
+ * per lane (one scalar, or one, two or four 16-byte vectors: how many lanes
+ * share a row) and the workgroups per CU (1, 2, 3, 4, 6, 8), and with
+ * flags & 1 also over the SELL-C-sigma layout (C = 16, 32, 64; sigma = 1,
+ * 256; the device copy is rebuilt from the host CRS for each).  The best
+ * layout (least total time over the widths) is kept, and the best launch
+ * settings of each width are stored on the matrix: later SpMVs of that width
+ * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu set global
+ * values.  Choices are cached for the process by matrix signature (scalar
+ * type, rows, entries, longest row, mean distance of entries from the
+ * diagonal, block width); flags & 2 measures again instead of using the
+ * cache; flags & 4 (D only) also tunes the matrix's FP32 copy used by the
+ * mixed-precision solvers (launch settings, same widths).  Outputs may be
+ * NULL: default_ms[i] / tuned_ms[i] = fastest of three products with the
+ * library's untuned choice on the incoming layout / with the chosen settings;
+ * settings[4*i..4*i+3] = C, sigma, lane bytes, workgroups per CU.  Products
+ * are identical bit for bit under every setting. */
+void phist_Damd_sparseMat_tune(phist_DsparseMat_ptr A, int n_nvec, const int* nvecs, int flags, double* default_ms,
+                               double* tuned_ms, int* settings, int* iflag);
+void phist_Samd_sparseMat_tune(phist_SsparseMat_ptr A, int n_nvec, const int* nvecs, int flags, double* default_ms,
+                               double* tuned_ms, int* settings, int* iflag);
 /* the device-resident MINRES iteration (blocked MINRES, the eigensolver's
  * correction equations, the service loop): iters_per_call iterations are
  * queued per host step, never more than an unfinished system may still take

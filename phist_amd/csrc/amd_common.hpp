@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <vector>
 #include <string>
 #include <stdexcept>
@@ -234,6 +235,15 @@ template <typename T> void sd_from_device_async(const SdMat<T>* M);
 // ----------------------------------------------------------------------------
 // sparse matrices
 // ----------------------------------------------------------------------------
+// launch settings of the SpMV for one matrix and block width: bytes each
+// lane gathers per nonzero (0 = one 16-byte vector) and workgroups per CU
+// (0 = as many as fit)
+struct SpmvTuned {
+  int lane_bytes = 0;
+  int blocks_per_cu = 0;
+  double ms = 0.0;  // measured time per product when chosen
+};
+
 template <typename T>
 struct SparseMat {
   Map* row_map = nullptr;  // owned
@@ -265,6 +275,9 @@ struct SparseMat {
   // FP32 copy of an FP64 matrix for mixed-precision solvers (made on first
   // use, freed with the matrix)
   SparseMat<float>* fp32_twin = nullptr;
+  // SpMV launch settings the autotuner chose for this matrix, by block width
+  // (tune.cpp); the SpMV uses them unless a global setting overrides them
+  std::map<int, SpmvTuned> tuned;
 };
 
 // the cached FP32 copy of A (same SELL layout, values rounded to float)
@@ -273,6 +286,13 @@ const SparseMat<float>* sparse_fp32_twin(const SparseMat<double>* A);
 // build the SELL-C-sigma device copy of a matrix whose host CRS is filled
 template <typename T> void sell_build(SparseMat<T>* A, int C, int sigma);
 template <typename T> void sparse_free(SparseMat<T>* A);
+// replace the device SELL copy by one with chunk height C and window sigma
+// (the FP32 twin and the tuned launch settings are dropped with the old one)
+template <typename T> void sell_rebuild(SparseMat<T>* A, int C, int sigma);
+// SpMV autotuner (tune.cpp): see phist_amd_ext.h
+template <typename T>
+void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, double* default_ms, double* tuned_ms,
+                 int* settings);
 
 // handle casts
 template <typename T> struct Handles;
@@ -318,6 +338,10 @@ struct SpmvRequest {
   int gram_ww_ld = 0;
   void* gram_vw_dev = nullptr;
   int gram_vw_ld = 0;
+  // launch settings for this call only (the autotuner's trials): lane bytes
+  // (>= 0) and workgroups per CU (>= 0); -1 = the usual choice
+  int lane_bytes = -1;
+  int blocks_per_cu = -1;
   bool no_store = false;
 };
 

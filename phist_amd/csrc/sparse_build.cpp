@@ -181,6 +181,24 @@ void sell_build(SparseMat<T>* A, int C, int sigma) {
 }
 
 template <typename T>
+void sell_rebuild(SparseMat<T>* A, int C, int sigma) {
+  sync_stream();
+  if (A->fp32_twin) {  // made again from the new layout on first use
+    sparse_free(A->fp32_twin);
+    A->fp32_twin = nullptr;
+  }
+  for (void* p : {(void*)A->d_chunk_ptr, (void*)A->d_chunk_w, (void*)A->d_perm, (void*)A->d_col, (void*)A->d_val})
+    if (p) PHX_HIP(hipFree(p));
+  A->d_chunk_ptr = nullptr;
+  A->d_chunk_w = nullptr;
+  A->d_perm = nullptr;
+  A->d_col = nullptr;
+  A->d_val = nullptr;
+  A->tuned.clear();  // launch settings belong to a layout
+  sell_build(A, C, sigma);
+}
+
+template <typename T>
 void sparse_free(SparseMat<T>* A) {
   if (!A) return;
   sync_stream();
@@ -382,6 +400,8 @@ SparseMat<T>* sparse_from_mm(const Comm* comm, const char* filename) {
 
 template void sell_build<double>(SparseMat<double>*, int, int);
 template void sell_build<float>(SparseMat<float>*, int, int);
+template void sell_rebuild<double>(SparseMat<double>*, int, int);
+template void sell_rebuild<float>(SparseMat<float>*, int, int);
 template void sparse_free<double>(SparseMat<double>*);
 template void sparse_free<float>(SparseMat<float>*);
 template SparseMat<double>* sparse_from_rowfunc<double>(const Comm*, int64_t, int64_t, int32_t, phist_sparseMat_rowFunc,

@@ -466,8 +466,13 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   if (y) vec_ok = vec_ok && (y->lda % V16 == 0) && aligned16(y->dev());
   if (req.axpy_u) vec_ok = vec_ok && (U->lda % V16 == 0) && aligned16(U->dev());
   int VPL = vec_ok ? V16 : 1;
+  const auto tuned = A->tuned.find(nvec);
   if (!gram) {  // the gathered bytes per lane and nonzero (same sums whichever is taken)
-    const int lb = rt().spmv_lane_bytes;
+    int lb = rt().spmv_lane_bytes;
+    if (req.lane_bytes >= 0)
+      lb = req.lane_bytes;
+    else if (lb == 0 && tuned != A->tuned.end())
+      lb = tuned->second.lane_bytes;
     const int w = lb / (int)sizeof(T);
     if (lb > 0 && lb <= (int)sizeof(T)) {
       VPL = 1;
@@ -500,9 +505,14 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   a.flags = (req.has_shift ? F_SHIFT : 0) | (req.has_beta ? F_BETA : 0) | (req.dot_ww ? F_DWW : 0) |
             (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0) | (req.gram_ww_dev ? F_GWW : 0) |
             (req.gram_vw_dev ? F_GVW : 0) | ((req.no_store || !y) ? F_NOSTORE : 0);
-  a.blocks_per_cu = rt().spmv_blocks_per_cu >= 0
-                        ? rt().spmv_blocks_per_cu
-                        : spmv_table_blocks_per_cu(A->nrows ? (double)A->nnz / (double)A->nrows : 0.0, nvec);
+  if (req.blocks_per_cu >= 0)
+    a.blocks_per_cu = req.blocks_per_cu;
+  else if (rt().spmv_blocks_per_cu >= 0)
+    a.blocks_per_cu = rt().spmv_blocks_per_cu;
+  else if (tuned != A->tuned.end())
+    a.blocks_per_cu = tuned->second.blocks_per_cu;
+  else
+    a.blocks_per_cu = spmv_table_blocks_per_cu(A->nrows ? (double)A->nnz / (double)A->nrows : 0.0, nvec);
   const bool dots = req.dot_ww || req.dot_vw;
 
   for (int cb = 0; cb < nvec; cb += maxcols) {

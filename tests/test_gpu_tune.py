@@ -1,0 +1,94 @@
+"""GPU: the SpMV autotuner (phist_[DS]amd_sparseMat_tune) through the C ABI.
+
+Checked: the settings it reports are valid choices; the product after tuning
+(new SELL layout and launch settings) equals the product before bit for bit
+and the oracle to the FP64 bar; the host CRS is untouched and the rebuilt
+SELL layout keeps its invariants; a second call with the cache returns the
+same settings; the FP32 copy can be tuned with the FP64 matrix.  Timings are
+reported, not asserted beyond a loose sanity bound (the untuned choice is
+one of the candidates)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _tune(be, A, nvecs, flags):
+    n = len(nvecs)
+    nv = np.array(nvecs, dtype=np.int32)
+    dms, tms = np.zeros(n), np.zeros(n)
+    st = np.zeros(4 * n, dtype=np.int32)
+    fl = ctypes.c_int(0)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    getattr(be.L, f"phist_{be.p}amd_sparseMat_tune")(A.h, ctypes.c_int(n), p(nv), ctypes.c_int(flags), p(dms),
+                                                      p(tms), p(st), ctypes.byref(fl))
+    assert fl.value == 0
+    return dms, tms, st.reshape(n, 4)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_tune_layout_and_launch(be64, be32, dtype):
+    be = be64 if dtype == np.float64 else be32
+    n = 150000
+    A = be.banded_random(n, 13, 4000, 40.0, 321)
+    ro, ci, va = A.crs()
+    ref = ol.Crs(n, n, ro, ci, va.astype(float))
+    m = be.map(n)
+    rng = np.random.default_rng(4)
+    before = {}
+    for nvec in (1, 8):
+        X = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+        x, y = be.mvec(m, nvec, X), be.mvec(m, nvec)
+        be.spmv(1.0, A, x, 0.0, y)
+        before[nvec] = (X, y.get())
+        x.delete()
+        y.delete()
+    dms, tms, st = _tune(be, A, [1, 8], 1 | 2)
+    item = np.dtype(dtype).itemsize
+    for (C, sigma, lb, bpc), d, t in zip(st, dms, tms):
+        assert C in (16, 32, 64) and sigma in (1, 256)
+        assert lb in (0, item, 32, 64) and bpc in (1, 2, 3, 4, 6, 8)
+        assert 0 < t <= 1.5 * d + 0.02, (t, d)
+    s = A.sell()
+    assert s["C"] == st[0][0]
+    perm = s["perm"]
+    assert sorted(perm[perm >= 0].tolist()) == list(range(n))
+    ro2, ci2, va2 = A.crs()
+    assert np.array_equal(ro, ro2) and np.array_equal(ci, ci2) and np.array_equal(va, va2)
+    for nvec, (X, Y0) in before.items():
+        x, y = be.mvec(m, nvec, X), be.mvec(m, nvec)
+        be.spmv(1.0, A, x, 0.0, y)
+        Y = y.get()
+        assert np.array_equal(Y, Y0), f"nvec {nvec}: tuned product differs"
+        want = ol.spmv(ref, 1.0, X.astype(float), 0.0, np.zeros((n, nvec)))
+        num = np.linalg.norm(Y.astype(float) - want, axis=0)
+        assert np.all(num <= (1e-12 if dtype == np.float64 else 1e-5) * np.linalg.norm(want, axis=0))
+        x.delete()
+        y.delete()
+    # the cache answers a second call with the same choice
+    _, _, st2 = _tune(be, A, [1, 8], 0)
+    assert np.array_equal(st2, st)
+    A.delete()
+    m.delete()
+
+
+def test_tune_fp32_twin(be64):
+    """flags & 4: the FP32 copy used by mixed-precision solvers is tuned too;
+    the FP64 matrix's products are unchanged"""
+    n = 60000
+    A = be64.banded_random(n, 9, 500, 40.0, 8)
+    m = be64.map(n)
+    X = np.random.default_rng(6).uniform(-1, 1, (n, 4))
+    x, y = be64.mvec(m, 4, X), be64.mvec(m, 4)
+    be64.spmv(1.0, A, x, 0.0, y)
+    Y0 = y.get()
+    _tune(be64, A, [4], 4 | 2)
+    be64.spmv(1.0, A, x, 0.0, y)
+    assert np.array_equal(y.get(), Y0)
+    for o in (x, y, A):
+        o.delete()
+    m.delete()
