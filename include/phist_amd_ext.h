@@ -88,6 +88,24 @@ void phist_amd_kernel_timer_read(double* total_ms, int64_t* count, int* iflag);
  * calibrates the L2 fetch counters for gathers of that size. */
 void phist_amd_bench_gather(int seg_bytes, double* mean_bw, double* max_bw, int* iflag);
 
+/* phist_Dfused_spmv_mvdot that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * W = alpha*A*V + beta*W as phist_Dfused_spmv_mvdot, with the column dots
+ * WdotW[j] = W_j.W_j and VdotW[j] = V_j.W_j written to DEVICE arrays of nvec
+ * doubles (either may be NULL), asynchronously on the library stream
+ * (phist_amd_get_stream): the call does not wait for the device.  Same kernels
+ * and summation order as phist_Dfused_spmv_mvdot, so W and the dots equal
+ * its results bit for bit.  The reduction partials live in a fixed device
+ * array of 2 MiB per GPU, held until the process ends (enough for 64 columns
+ * on the persistent grid of up to 2047 workgroups); a call needing more, a
+ * block of more than 64 columns or an empty block fails with -55
+ * (PHIST_INVALID_INPUT) before anything is launched.  Every call on one GPU
+ * uses the same array: calls ordered on the library stream are safe, calls
+ * that overlap on one GPU (several host threads) race. */
+void phist_Dfused_spmv_mvdot_noalloc(double alpha, phist_Dconst_sparseMat_ptr A, phist_Dconst_mvec_ptr V,
+                                     double beta, phist_Dmvec_ptr W, double* WdotW, double* VdotW, int* iflag);
+
 /* the global KISS stream: reset to the initial state, jump ahead by n draws,
  * draw n doubles in [-1,1) */
 void phist_amd_kiss_reset(int* iflag);

@@ -135,6 +135,14 @@ struct Runtime {
   bool capturing = false;
   void* work_arena = nullptr;
   size_t work_need = 0;
+  // the allocation-free entry points: address of the fixed device scratch
+  // array on each device (hipGetSymbolAddress at initialisation), and while
+  // such a call runs, the buffer device_work answers with (never allocating;
+  // larger requests are rejected) -- host_work is refused meanwhile
+  static constexpr int kMaxDevices = 64;
+  void* noalloc_scratch[kMaxDevices] = {};
+  void* fixed_work = nullptr;
+  size_t fixed_work_bytes = 0;
   bool gram_rows = true;  // narrow unaligned Grams: several columns per lane (k_gram_rows)
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
@@ -148,6 +156,18 @@ void ensure_init();
 void* device_work(size_t bytes);
 void* host_work(size_t bytes);
 void sync_stream();
+// Fixed device scratch of the allocation-free entry points: kNoAllocScratchBytes
+// per device, held until the process ends.  noalloc_scratch() returns this
+// device's address of it (resolved at initialisation).  NoAllocScope makes
+// device_work answer every request from it for the scope's lifetime (a
+// request larger than the array fails with PHIST_INVALID_INPUT before
+// anything is launched) and host_work fail.
+constexpr size_t kNoAllocScratchBytes = (size_t)2 << 20;
+void* noalloc_scratch();
+struct NoAllocScope {
+  NoAllocScope();
+  ~NoAllocScope();
+};
 // Queues a 2-D host-to-device copy (cols columns of row_bytes bytes, pitches
 // in bytes) from a pinned staging slot holding a copy of the source, so the
 // caller may change or free its array at once; false when the block does not
@@ -319,6 +339,10 @@ struct SpmvRequest {
   // when set, the dots stay on the device: dots_dev[0..nvec) = ww, dots_dev[nvec..2nvec) = vw
   // (both written whichever were requested; at most 64 column groups)
   void* dots_dev = nullptr;
+  // when set (instead of dots_dev), the dots stay on the device in two
+  // arrays of nvec entries: dot_ww_dev = ww, dot_vw_dev = vw (either may be null)
+  void* dot_ww_dev = nullptr;
+  void* dot_vw_dev = nullptr;
   // when set (with has_shift), the shifts are read on the device from
   // shifts_dev[0..nvec) instead of the host array
   const void* shifts_dev = nullptr;

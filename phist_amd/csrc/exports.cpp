@@ -42,3 +42,33 @@ extern "C" void phist_amd_Scarp_colors(void* work, int* ncolors, int64_t* color_
     *iflag = 0;
   });
 }
+
+// fused_spmv_mvdot that allocates nothing: the dot results stay on the device
+// and the reduction partials live in the library's fixed device scratch
+extern "C" void phist_Dfused_spmv_mvdot_noalloc(double alpha, phist_Dconst_sparseMat_ptr A, phist_Dconst_mvec_ptr V,
+                                                double beta, phist_Dmvec_ptr W, double* WdotW_dev, double* VdotW_dev,
+                                                int* iflag) {
+  guard(iflag, "phist_Dfused_spmv_mvdot_noalloc", [&] {
+    const SparseMat<double>* a = SP<double>(A);
+    const Mvec<double>* v = MV<double>(V);
+    Mvec<double>* w = MV<double>(W);
+    PHX_REQUIRE(v && w, PHIST_INVALID_INPUT, "V and W are required");
+    PHX_REQUIRE(v->nvec <= 64, PHIST_INVALID_INPUT, "at most 64 columns (V has %d)", v->nvec);
+    PHX_REQUIRE(w->nrows > 0, PHIST_INVALID_INPUT, "empty block");
+    NoAllocScope scope;
+    if (spmv_trivial(a, alpha)) {
+      blas_scale_cols<double>(w, &beta, true);
+      if (VdotW_dev) blas_dot_dev<double>(v, w, VdotW_dev);
+      if (WdotW_dev) blas_dot_dev<double>(w, w, WdotW_dev);
+    } else {
+      SpmvRequest rq;
+      rq.has_beta = beta != 0.0;
+      rq.dot_ww = WdotW_dev != nullptr;
+      rq.dot_vw = VdotW_dev != nullptr;
+      rq.dot_ww_dev = WdotW_dev;
+      rq.dot_vw_dev = VdotW_dev;
+      spmv_launch<double>(a, alpha, nullptr, v, beta, w, rq, nullptr, nullptr);
+    }
+    *iflag = 0;
+  });
+}

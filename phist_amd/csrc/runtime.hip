@@ -29,6 +29,41 @@ Runtime& rt() {
   return r;
 }
 
+// Device scratch of the allocation-free entry points (reduction partials of
+// one call).  Host code never names it after initialisation: kernels and
+// copies get the address hipGetSymbolAddress returned for the device.
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+static void resolve_noalloc_scratch(Runtime& r) {
+  PHX_REQUIRE(r.device >= 0 && r.device < Runtime::kMaxDevices, PHIST_FUNCTIONAL_ERROR,
+              "device %d beyond the %d devices the fixed scratch table holds", r.device, Runtime::kMaxDevices);
+  if (r.noalloc_scratch[r.device]) return;
+  void* p = nullptr;
+  PHX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  PHX_REQUIRE(p != nullptr, PHIST_FUNCTIONAL_ERROR, "no device address for the fixed scratch array");
+  r.noalloc_scratch[r.device] = p;
+}
+
+void* noalloc_scratch() {
+  Runtime& r = rt();
+  PHX_REQUIRE(r.initialized, PHIST_FUNCTIONAL_ERROR, "library not initialised");
+  void* p = r.noalloc_scratch[r.device];
+  PHX_REQUIRE(p != nullptr, PHIST_FUNCTIONAL_ERROR, "fixed scratch of device %d not resolved", r.device);
+  return p;
+}
+
+NoAllocScope::NoAllocScope() {
+  Runtime& r = rt();
+  PHX_REQUIRE(!r.capturing, PHIST_FUNCTIONAL_ERROR, "allocation-free call inside the library's own graph capture");
+  r.fixed_work = noalloc_scratch();
+  r.fixed_work_bytes = kNoAllocScratchBytes;
+}
+NoAllocScope::~NoAllocScope() {
+  Runtime& r = rt();
+  r.fixed_work = nullptr;
+  r.fixed_work_bytes = 0;
+}
+
 void ensure_init() {
   Runtime& r = rt();
   if (r.initialized) return;
@@ -62,12 +97,20 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH")) r.minres_graphs = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_ITERS_PER_CALL")) r.minres_iters_per_call = std::max(0, atoi(s));
   if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH_LEN")) r.minres_graph_len = std::max(1, atoi(s));
+  PHX_HIP(hipSetDevice(r.device));
+  resolve_noalloc_scratch(r);
   kiss_reset();
   r.initialized = true;
 }
 
 void* device_work(size_t bytes) {
   Runtime& r = rt();
+  if (r.fixed_work) {
+    PHX_REQUIRE(bytes <= r.fixed_work_bytes, PHIST_INVALID_INPUT,
+                "the call needs %zu bytes of device scratch, the allocation-free entry point has %zu", bytes,
+                r.fixed_work_bytes);
+    return r.fixed_work;
+  }
   if (r.capturing) {
     r.work_need = std::max(r.work_need, bytes);
     return r.work_arena;
@@ -87,6 +130,7 @@ void* device_work(size_t bytes) {
 
 void* host_work(size_t bytes) {
   Runtime& r = rt();
+  PHX_REQUIRE(!r.fixed_work, PHIST_FUNCTIONAL_ERROR, "host-side reduction inside an allocation-free call");
   PHX_REQUIRE(!r.capturing, PHIST_FUNCTIONAL_ERROR, "a host-side reduction cannot be part of a captured graph");
   if (bytes > r.h_work_bytes) {
     if (r.h_work) {

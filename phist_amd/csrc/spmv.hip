@@ -554,6 +554,18 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
                          grid, 2 * nc, static_cast<T*>(req.dots_dev));
       PHX_HIP(hipGetLastError());
+    } else if (dots && (req.dot_ww_dev || req.dot_vw_dev)) {
+      // fixed-order sums into the tail of the partials buffer (sized for them), then to the two outputs
+      T* sums = a.partials + (size_t)grid * 2 * nc;
+      hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
+                         grid, 2 * nc, sums);
+      PHX_HIP(hipGetLastError());
+      if (req.dot_ww_dev)
+        PHX_HIP(hipMemcpyAsync(static_cast<T*>(req.dot_ww_dev) + cb, sums, sizeof(T) * nc, hipMemcpyDeviceToDevice,
+                               rt().stream));
+      if (req.dot_vw_dev)
+        PHX_HIP(hipMemcpyAsync(static_cast<T*>(req.dot_vw_dev) + cb, sums + nc, sizeof(T) * nc,
+                               hipMemcpyDeviceToDevice, rt().stream));
     } else if (dots) {
       std::vector<T> sums(2 * nc);
       reduce_partials_to_host(a.partials, grid, 2 * nc, sums.data());
