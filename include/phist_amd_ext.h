@@ -35,15 +35,24 @@ void phist_amd_set_spmv_blocks_per_cu(int n, int* iflag);
  * the same result bit for bit: a row's sum is always formed by the lane that
  * owns its columns, in the row's entry order. */
 void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag);
+/* entries of a row whose val/col loads and x gathers the SpMV issues
+ * together before their multiply-adds (0 = the default: the matrix's tuned
+ * value, else 8; 8, 16 or 32).  16 and 32 also request the next batch's
+ * val/col before the current batch's gathers; they apply to row groups of at
+ * most 8 lanes gathering at most 32 (depth 16) or 16 bytes (depth 32) each,
+ * without per-column x factors, and fall back to 8 otherwise.  Every choice
+ * gives the same result bit for bit. */
+void phist_amd_set_spmv_depth(int slots, int* iflag);
 /* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16
  * widths) it times the product y = A*x on the device over the gather width
  * per lane (one scalar, or one, two or four 16-byte vectors: how many lanes
- * share a row) and the workgroups per CU (1, 2, 3, 4, 6, 8), and with
+ * share a row), the workgroups per CU (1, 2, 3, 4, 6, 8) and, for rows of at
+ * least 16 entries, the depth (8, 16 or 32, phist_amd_set_spmv_depth), and with
  * flags & 1 also over the SELL-C-sigma layout (C = 16, 32, 64; sigma = 1,
  * 256; the device copy is rebuilt from the host CRS for each).  The best
  * layout (least total time over the widths) is kept, and the best launch
  * settings of each width are stored on the matrix: later SpMVs of that width
- * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu set global
+ * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu / _depth set global
  * values.  Choices are cached for the process by matrix signature (scalar
  * type, rows, entries, longest row, mean distance of entries from the
  * diagonal, block width); flags & 2 measures again instead of using the
@@ -51,7 +60,7 @@ void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag);
  * mixed-precision solvers (launch settings, same widths).  Outputs may be
  * NULL: default_ms[i] / tuned_ms[i] = fastest of three products with the
  * library's untuned choice on the incoming layout / with the chosen settings;
- * settings[4*i..4*i+3] = C, sigma, lane bytes, workgroups per CU.  Products
+ * settings[5*i..5*i+4] = C, sigma, lane bytes, workgroups per CU, depth.  Products
  * are identical bit for bit under every setting. */
 void phist_Damd_sparseMat_tune(phist_DsparseMat_ptr A, int n_nvec, const int* nvecs, int flags, double* default_ms,
                                double* tuned_ms, int* settings, int* iflag);

@@ -98,6 +98,9 @@ struct Runtime {
   int spmv_blocks_per_cu = -1;  // < 0: spmv_table_blocks_per_cu
   // bytes of a block row each SpMV lane gathers per nonzero (0: one 16-byte vector)
   int spmv_lane_bytes = 0;
+  // entries per row whose val/col loads and x gathers the SpMV issues
+  // together (8 or 32; < 0: the matrix's tuned value, else 8)
+  int spmv_depth = -1;
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
   // ... when the wider side has at least gram_mfma_wide columns, or an
@@ -258,9 +261,23 @@ template <typename T> void sd_from_device_async(const SdMat<T>* M);
 // launch settings of the SpMV for one matrix and block width: bytes each
 // lane gathers per nonzero (0 = one 16-byte vector) and workgroups per CU
 // (0 = as many as fit)
+// the depth the SpMV runs for a requested depth (8, 16 or 32), the scalars
+// per lane vpl, the scalar size and the block width: 32 needs at most 16
+// bytes per lane, 16 at most 32, and both at most 8 lanes per row (blocks of
+// at most 64 columns' lane groups); otherwise the next smaller depth
+inline int spmv_effective_depth(int depth, int vpl, int scalar_bytes, int ncols) {
+  int groups = (std::min(ncols, 64 * vpl) + vpl - 1) / vpl, lpr = 1;
+  while (lpr < groups) lpr <<= 1;
+  if (lpr > 8) return 8;
+  if (depth >= 32 && vpl * scalar_bytes <= 16) return 32;
+  if (depth >= 16 && vpl * scalar_bytes <= 32) return 16;
+  return 8;
+}
+
 struct SpmvTuned {
   int lane_bytes = 0;
   int blocks_per_cu = 0;
+  int depth = 8;  // entries per row whose loads are issued together (8 or 32)
   double ms = 0.0;  // measured time per product when chosen
 };
 
@@ -366,6 +383,7 @@ struct SpmvRequest {
   // (>= 0) and workgroups per CU (>= 0); -1 = the usual choice
   int lane_bytes = -1;
   int blocks_per_cu = -1;
+  int depth = -1;
   bool no_store = false;
 };
 

@@ -85,6 +85,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_GRID_BLOCKS")) r.grid_blocks = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_LANE_BYTES")) r.spmv_lane_bytes = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_SPMV_DEPTH")) r.spmv_depth = atoi(s);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA_MIN_M")) r.mix_mfma_min_m = std::max(1, atoi(s));
@@ -646,6 +647,15 @@ void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag) {
     PHX_REQUIRE(bytes == 0 || bytes == 4 || bytes == 8 || bytes == 16 || bytes == 32 || bytes == 64,
                 PHIST_INVALID_INPUT, "lane width %d bytes: 0, 4, 8, 16, 32 or 64", bytes);
     rt().spmv_lane_bytes = bytes;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_spmv_depth(int slots, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_REQUIRE(slots == 0 || slots == 8 || slots == 16 || slots == 32, PHIST_INVALID_INPUT,
+                "depth %d: 0, 8, 16 or 32", slots);
+    rt().spmv_depth = slots == 0 ? -1 : slots;
     *iflag = 0;
   });
 }

@@ -128,7 +128,10 @@ __device__ inline T group_bcast_t(T v, int src, int lane) {
 // One row per LPR lanes; the lanes of a row load the row's val/col
 // cooperatively (lane q takes slot j = jb*LPR + q, coalesced over the rows
 // of the chunk) and share them by group broadcast, so every index is read
-// from memory once.  KB loads per lane are in flight per batch.
+// from memory once.  A batch covers SL entries of each row (the depth: 8, or
+// 32 for long rows): its KB = SL/LPR val/col loads per lane, then all its x
+// gathers, go out before the first FMA of the batch, so a row of 32 entries
+// costs one round trip for its indices and one for its gathers.
 //
 // GR > 0 (fused_spmv_mvTmv, at most 32 columns, GR = 16-column tiles): after
 // each tile the wave writes its rows of the new W (and of V) to LDS and
@@ -137,13 +140,14 @@ __device__ inline T group_bcast_t(T v, int src, int lane) {
 // registers; the wave tiles are added in LDS in wave order and the
 // workgroup tiles by a second fixed-order pass, so the sums are
 // deterministic.  W itself is not stored with F_NOSTORE (W = NULL).
-template <typename T, int VPL, int LPR, bool XSC, int GR>
+template <typename T, int VPL, int LPR, bool XSC, int GR, int SL>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   constexpr int RPW = 64 / LPR;
   constexpr int RPB = (kBlock / 64) * RPW;
-  constexpr int KB = LPR >= 16 ? 1 : (LPR >= 4 ? 2 : (LPR == 2 ? 4 : 8));
-  constexpr int BATCH = KB * LPR;  // slots per batch and row
-  constexpr int SUB = BATCH < 16 ? BATCH : 16;  // gathers in flight per lane
+  constexpr int KB = SL > LPR ? SL / LPR : 1;  // val/col loads per lane and batch
+  constexpr int BATCH = KB * LPR;              // slots per batch and row
+  constexpr int SUBMAX = SL >= 32 ? 32 : 16;
+  constexpr int SUB = BATCH < SUBMAX ? BATCH : SUBMAX;  // gathers issued before their FMAs
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int q = lane % LPR;
@@ -196,22 +200,9 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < VPL; e++) acc[e] = (flags & F_SHIFT) ? sh[e] * xo[e] : T(0);
 
-    for (int j0 = 0; j0 < wmax; j0 += BATCH) {
-      T v[KB];
-      int32_t cj[KB];
-#pragma unroll
-      for (int u = 0; u < KB; u++) {
-        const int j = j0 + u * LPR + q;
-        if (j < w) {
-          const int64_t slot = base + (int64_t)j * C;
-          v[u] = __builtin_nontemporal_load(a.val + slot);
-          cj[u] = __builtin_nontemporal_load(a.col + slot);
-        } else {
-          v[u] = T(0);
-          cj[u] = 0;
-        }
-      }
-      // slot j0+k of this row sits in lane k % LPR of the group, register k / LPR
+    // the batch's x gathers, then its FMAs in entry order: slot j0+k of this
+    // row sits in lane k % LPR of the group, register k / LPR
+    auto spmv_batch = [&](int j0, const T (&v)[KB], const int32_t (&cj)[KB]) {
 #pragma unroll
       for (int k0 = 0; k0 < BATCH; k0 += SUB) {
         T vv[SUB];
@@ -231,6 +222,80 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 #pragma unroll
             for (int e = 0; e < VPL; e++) acc[e] = fma(vv[kk], xg[kk][e], acc[e]);
           }
+      }
+    };
+    // val/col of the batch starting at entry j0 (0 / column 0 beyond the row)
+    auto load_batch = [&](int j0, T (&v)[KB], int32_t (&cj)[KB]) {
+#pragma unroll
+      for (int u = 0; u < KB; u++) {
+        const int j = j0 + u * LPR + q;
+        if (j < w) {
+          const int64_t slot = base + (int64_t)j * C;
+          v[u] = __builtin_nontemporal_load(a.val + slot);
+          cj[u] = __builtin_nontemporal_load(a.col + slot);
+        } else {
+          v[u] = T(0);
+          cj[u] = 0;
+        }
+      }
+    };
+    // depth >= 16: the next batch's val/col are requested before this
+    // batch's gathers, so the index loads overlap the gathers; depth 8
+    // loads each batch's indices, then gathers
+    if constexpr (SL >= 16) {
+      T v[KB];
+      int32_t cj[KB];
+      if (wmax > 0) load_batch(0, v, cj);
+      for (int j0 = 0; j0 < wmax; j0 += BATCH) {
+        T vn[KB];
+        int32_t cn[KB];
+        const bool more = j0 + BATCH < wmax;  // uniform over the wave
+        if (more) load_batch(j0 + BATCH, vn, cn);
+        spmv_batch(j0, v, cj);
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < KB; u++) {
+            v[u] = vn[u];
+            cj[u] = cn[u];
+          }
+        }
+      }
+    } else {
+      for (int j0 = 0; j0 < wmax; j0 += BATCH) {
+        T v[KB];
+        int32_t cj[KB];
+#pragma unroll
+        for (int u = 0; u < KB; u++) {
+          const int j = j0 + u * LPR + q;
+          if (j < w) {
+            const int64_t slot = base + (int64_t)j * C;
+            v[u] = __builtin_nontemporal_load(a.val + slot);
+            cj[u] = __builtin_nontemporal_load(a.col + slot);
+          } else {
+            v[u] = T(0);
+            cj[u] = 0;
+          }
+        }
+#pragma unroll
+        for (int k0 = 0; k0 < BATCH; k0 += SUB) {
+          T vv[SUB];
+          T xg[SUB][VPL];
+          bool ok[SUB];
+#pragma unroll
+          for (int kk = 0; kk < SUB; kk++) {
+            const int k = k0 + kk;
+            const int32_t cc = group_bcast<LPR>(cj[k / LPR], k % LPR, lane);
+            vv[kk] = group_bcast_t<LPR>(v[k / LPR], k % LPR, lane);
+            ok[kk] = colok && (j0 + k) < w;
+            if (ok[kk]) vload<T, VPL>(xb + (int64_t)cc * a.ldx, xg[kk]);
+          }
+#pragma unroll
+          for (int kk = 0; kk < SUB; kk++)
+            if (ok[kk]) {
+#pragma unroll
+              for (int e = 0; e < VPL; e++) acc[e] = fma(vv[kk], xg[kk][e], acc[e]);
+            }
+        }
       }
     }
     T yn[VPL];
@@ -371,9 +436,9 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 
 namespace {
 
-template <typename T, int VPL, int LPR, bool XSC, int GR = 0>
+template <typename T, int VPL, int LPR, bool XSC, int GR = 0, int SL = 8>
 int launch_one(SpmvArgs<T>& a) {
-  auto kern = k_sell_spmv<T, VPL, LPR, XSC, GR>;
+  auto kern = k_sell_spmv<T, VPL, LPR, XSC, GR, SL>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   a.ntiles = (a.nslots + RPB - 1) / RPB;
   const int grid = persistent_blocks(kern, 0, a.ntiles, a.blocks_per_cu);
@@ -400,9 +465,28 @@ int launch_lpr(SpmvArgs<T>& a, int lpr) {
     default: return launch_one<T, VPL, 64, XSC>(a);
   }
 }
+// depth 16 or 32 (rows of many entries; no per-column x factors): row groups of at most 8 lanes
+template <typename T, int VPL, int SL>
+int launch_deep(SpmvArgs<T>& a, int lpr) {
+  switch (lpr) {
+    case 1: return launch_one<T, VPL, 1, false, 0, SL>(a);
+    case 2: return launch_one<T, VPL, 2, false, 0, SL>(a);
+    case 4: return launch_one<T, VPL, 4, false, 0, SL>(a);
+    default: return launch_one<T, VPL, 8, false, 0, SL>(a);
+  }
+}
 
+// the deeper batches hold depth * VPL gathered values per lane: depth 32
+// takes at most one 16-byte vector per lane, depth 16 at most two (wider
+// lanes would need more registers than the file holds without spilling)
 template <typename T, int VPL>
-int launch_xsc(SpmvArgs<T>& a, int lpr) {
+int launch_xsc(SpmvArgs<T>& a, int lpr, int depth) {
+  if (!a.xscale && lpr <= 8) {
+    if constexpr (VPL * sizeof(T) <= 16)
+      if (depth >= 32) return launch_deep<T, VPL, 32>(a, lpr);
+    if constexpr (VPL * sizeof(T) <= 32)
+      if (depth >= 16) return launch_deep<T, VPL, 16>(a, lpr);
+  }
   return a.xscale ? launch_lpr<T, VPL, true>(a, lpr) : launch_lpr<T, VPL, false>(a, lpr);
 }
 
@@ -513,6 +597,16 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
     a.blocks_per_cu = tuned->second.blocks_per_cu;
   else
     a.blocks_per_cu = spmv_table_blocks_per_cu(A->nrows ? (double)A->nnz / (double)A->nrows : 0.0, nvec);
+  // entries per row whose loads go out together (8 or 32)
+  int depth;
+  if (req.depth >= 0)
+    depth = req.depth;
+  else if (rt().spmv_depth >= 0)
+    depth = rt().spmv_depth;
+  else if (tuned != A->tuned.end())
+    depth = tuned->second.depth;
+  else
+    depth = 8;
   const bool dots = req.dot_ww || req.dot_vw;
 
   for (int cb = 0; cb < nvec; cb += maxcols) {
@@ -545,10 +639,10 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
       }
       continue;
     }
-    const int grid = VPL == 1     ? launch_xsc<T, 1>(a, lpr)
-                     : VPL == V16 ? launch_xsc<T, V16>(a, lpr)
-                     : VPL == 2 * V16 ? launch_xsc<T, 2 * V16>(a, lpr)
-                                      : launch_xsc<T, 4 * V16>(a, lpr);
+    const int grid = VPL == 1     ? launch_xsc<T, 1>(a, lpr, depth)
+                     : VPL == V16 ? launch_xsc<T, V16>(a, lpr, depth)
+                     : VPL == 2 * V16 ? launch_xsc<T, 2 * V16>(a, lpr, depth)
+                                      : launch_xsc<T, 4 * V16>(a, lpr, depth);
     if (dots && req.dots_dev) {
       PHX_REQUIRE(cb == 0 && nc == nvec, PHIST_INVALID_INPUT, "device dots need one column block (%d columns)", nvec);
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,

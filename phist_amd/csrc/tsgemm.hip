@@ -41,6 +41,12 @@ struct GramArgs {
   T* __restrict__ partials;  // [grid][na*nb], index c*na + r (r over a, c over b)
 };
 
+// two consecutive scalars as one 16-byte (FP64) / 8-byte (FP32) load
+template <typename T>
+struct alignas(2 * sizeof(T)) T2v {
+  T x, y;
+};
+
 template <typename T, int VPL, int LPR, int KB>
 __global__ void __launch_bounds__(kBlock) k_gram(const GramArgs<T> g) {
   constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
@@ -53,25 +59,36 @@ __global__ void __launch_bounds__(kBlock) k_gram(const GramArgs<T> g) {
 #pragma unroll
     for (int c = 0; c < KB; c++) acc[e][c] = T(0);
 
+  // UNR tiles per iteration: their loads are all in flight before the first
+  // shuffle; the tiles are accumulated in walk order, as one at a time would
+  constexpr int UNR = 4;
+  const bool self = g.a == g.b && g.lda == g.ldb && g.na == g.nb;  // the b row is the a row
   const TileWalk tw = xcd_tiles(g.ntiles);
-  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
-    const int64_t i = t * RPB + wv * RPW + rw;
-    const bool rowok = i < g.nrows;  // all lanes stay for the shuffles
-    T av[VPL], bv[VPL];
+  for (int64_t t = tw.t; t < tw.end; t += UNR * tw.stride) {
+    T av[UNR][VPL], bv[UNR][VPL];
 #pragma unroll
-    for (int e = 0; e < VPL; e++) av[e] = bv[e] = T(0);
-    if (rowok && c0 < g.na) vload<T, VPL>(g.a + i * g.lda + c0, av);
-    if (rowok && c0 < g.nb) vload<T, VPL>(g.b + i * g.ldb + c0, bv);
+    for (int u = 0; u < UNR; u++) {
+      const int64_t tu = t + u * tw.stride;
+      const int64_t i = tu * RPB + wv * RPW + rw;
+      const bool rowok = tu < tw.end && i < g.nrows;  // all lanes stay for the shuffles
 #pragma unroll
-    for (int s = 0; s < LPR; s++) {
-      if (s * VPL >= KB) break;
+      for (int e = 0; e < VPL; e++) av[u][e] = bv[u][e] = T(0);
+      if (rowok && c0 < g.na) vload<T, VPL>(g.a + i * g.lda + c0, av[u]);
+      if (!self && rowok && c0 < g.nb) vload<T, VPL>(g.b + i * g.ldb + c0, bv[u]);
+    }
 #pragma unroll
-      for (int f = 0; f < VPL; f++) {
-        const int c = s * VPL + f;
-        if (c >= KB) break;
-        const T w = __shfl(bv[f], grp + s, 64);
+    for (int u = 0; u < UNR; u++) {
 #pragma unroll
-        for (int e = 0; e < VPL; e++) acc[e][c] = fma(av[e], w, acc[e][c]);
+      for (int s = 0; s < LPR; s++) {
+        if (s * VPL >= KB) break;
+#pragma unroll
+        for (int f = 0; f < VPL; f++) {
+          const int c = s * VPL + f;
+          if (c >= KB) break;
+          const T w = __shfl(self ? av[u][f] : bv[u][f], grp + s, 64);
+#pragma unroll
+          for (int e = 0; e < VPL; e++) acc[e][c] = fma(av[u][e], w, acc[e][c]);
+        }
       }
     }
   }
@@ -153,48 +170,76 @@ struct Mfma<float> {
 };
 
 // S = a'b for na, nb <= 32 (TM = ceil(na/16), TK = ceil(nb/16)); a wave
-// consumes 16 rows per iteration (four 4-row MFMA steps); with SELF (a == b)
-// only the upper block tiles are computed and mirrored.
-template <typename T, int TM, int TK, bool SELF>
+// consumes 16 rows per tile (four 4-row MFMA steps), two tiles per
+// iteration so that both tiles' loads are in flight together; with SELF
+// (a == b) only the upper block tiles are computed and mirrored.  With PAIR
+// (TM = 2, and TK = 2 unless SELF; even widths, 16-byte aligned rows) lane
+// cc loads columns 2cc and 2cc+1 of its row as one 16-byte vector: tile 0
+// holds the even columns, tile 1 the odd ones, so column (tile t, index c)
+// is 2c + t instead of 16t + c.  Every output is the same sum of the same
+// products in the same MFMA steps either way.
+template <typename T, int TM, int TK, bool SELF, bool PAIR = false>
 __global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
   using M = Mfma<T>;
+  static_assert(!PAIR || (TM == 2 && (SELF || TK == 2)), "paired loads need two column tiles per operand");
+  constexpr int UNR = 2;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kk = lane >> 4, cc = lane & 15;
+  auto colof = [&](int t, int c) { return PAIR ? 2 * c + t : 16 * t + c; };
   typename M::acc_t acc[TM][TK];
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
     for (int j = 0; j < TK; j++) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
   const TileWalk tw = xcd_tiles(g.ntiles);
-  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
-    const int64_t row0 = t * 64 + wv * 16;
-    T a[4][TM], b[4][TK];
+  for (int64_t t = tw.t; t < tw.end; t += UNR * tw.stride) {
+    T a[UNR][4][TM], b[UNR][4][TK];
 #pragma unroll
-    for (int st = 0; st < 4; st++) {
-      const int64_t r = row0 + st * 4 + kk;
-      const bool ok = r < g.nrows;
+    for (int u = 0; u < UNR; u++) {
+      const int64_t tu = t + u * tw.stride;
+      const int64_t row0 = tu * 64 + wv * 16;
 #pragma unroll
-      for (int tm = 0; tm < TM; tm++) {
-        const int col = tm * 16 + cc;
-        a[st][tm] = (ok && col < g.na) ? g.a[r * g.lda + col] : T(0);
-      }
-      if constexpr (!SELF) {
+      for (int st = 0; st < 4; st++) {
+        const int64_t r = row0 + st * 4 + kk;
+        const bool ok = tu < tw.end && r < g.nrows;
+        if constexpr (PAIR) {
+          T2v<T> av{T(0), T(0)};
+          if (ok && 2 * cc < g.na) av = *reinterpret_cast<const T2v<T>*>(g.a + r * g.lda + 2 * cc);
+          a[u][st][0] = av.x;
+          a[u][st][1] = av.y;
+          if constexpr (!SELF) {
+            T2v<T> bv{T(0), T(0)};
+            if (ok && 2 * cc < g.nb) bv = *reinterpret_cast<const T2v<T>*>(g.b + r * g.ldb + 2 * cc);
+            b[u][st][0] = bv.x;
+            b[u][st][1] = bv.y;
+          }
+        } else {
 #pragma unroll
-        for (int tk = 0; tk < TK; tk++) {
-          const int col = tk * 16 + cc;
-          b[st][tk] = (ok && col < g.nb) ? g.b[r * g.ldb + col] : T(0);
+          for (int tm = 0; tm < TM; tm++) {
+            const int col = tm * 16 + cc;
+            a[u][st][tm] = (ok && col < g.na) ? g.a[r * g.lda + col] : T(0);
+          }
+          if constexpr (!SELF) {
+#pragma unroll
+            for (int tk = 0; tk < TK; tk++) {
+              const int col = tk * 16 + cc;
+              b[u][st][tk] = (ok && col < g.nb) ? g.b[r * g.ldb + col] : T(0);
+            }
+          }
         }
       }
     }
 #pragma unroll
-    for (int st = 0; st < 4; st++)
+    for (int u = 0; u < UNR; u++)
 #pragma unroll
-      for (int tm = 0; tm < TM; tm++)
+      for (int st = 0; st < 4; st++)
 #pragma unroll
-        for (int tk = 0; tk < TK; tk++) {
-          if (SELF && tm > tk) continue;
-          acc[tm][tk] = M::op(a[st][tm], SELF ? a[st][tk] : b[st][tk], acc[tm][tk]);
-        }
+        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+          for (int tk = 0; tk < TK; tk++) {
+            if (SELF && tm > tk) continue;
+            acc[tm][tk] = M::op(a[u][st][tm], SELF ? a[u][st][tk] : b[u][st][tk], acc[tm][tk]);
+          }
   }
   extern __shared__ char gram_lds[];
   const int nout = g.na * g.nb;
@@ -206,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
         if (SELF && tm > tk) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = tm * 16 + M::row(lane, r), j = tk * 16 + cc;
+          const int i = colof(tm, M::row(lane, r)), j = colof(tk, cc);
           const T v = acc[tm][tk][r];
           if (i < g.na && j < g.nb) add(j * g.na + i, v);
           if (SELF && tm < tk && j < g.na && i < g.nb) add(i * g.na + j, v);
@@ -216,21 +261,31 @@ __global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
 }
 
 // C(r,c) = alpha * S + beta * C with S summed over the block partials in a
-// fixed order; S is stored c*na + r, transposed when the operands were swapped
+// fixed order; S is stored c*na + r, transposed when the operands were
+// swapped.  One wave per output (four per workgroup, finalize_blocks(nout)
+// workgroups): lane l adds the partials of blocks l, l+64, l+128, ... in
+// order, then the 64 lane sums are added by a fixed shuffle tree, so every
+// block's loads are in flight at once and the sum is deterministic.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_gram_finalize(const T* __restrict__ partials, int nblocks, int na, int nb,
                                                           bool transposed, T alpha, T beta, T* __restrict__ C,
                                                           int ldc) {
   const int nout = na * nb;
-  const T s = block_sum16(partials, nblocks, nout);
-  const int o = blockIdx.x * 16 + threadIdx.x;
-  if (threadIdx.x < 16 && o < nout) {
+  const int o = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (o >= nout) return;  // whole waves leave; no workgroup barrier follows
+  T s = T(0);
+#pragma unroll 8
+  for (int b = lane; b < nblocks; b += 64) s += partials[(int64_t)b * nout + o];
+  s = wave_sum_strided<1>(s);
+  if (lane == 0) {
     const int r = o % na, c = o / na;
     const int ci = transposed ? r : c, ri = transposed ? c : r;
     T* cp = C + (int64_t)ci * ldc + ri;
     *cp = beta == T(0) ? alpha * s : alpha * s + beta * *cp;
   }
 }
+inline int finalize_blocks(int nout) { return (nout + kBlock / 64 - 1) / (kBlock / 64); }
 
 namespace {
 
@@ -288,9 +343,9 @@ int gram_lpr(GramArgs<T>& g, int lpr) {
 
 }  // namespace
 
-template <typename T, int TM, int TK, bool SELF>
+template <typename T, int TM, int TK, bool SELF, bool PAIR = false>
 static int gram_mfma_launch(GramArgs<T>& g) {
-  auto kern = k_gram_mfma<T, TM, TK, SELF>;
+  auto kern = k_gram_mfma<T, TM, TK, SELF, PAIR>;
   g.ntiles = (g.nrows + 63) / 64;
   const size_t lds = sizeof(T) * g.na * g.nb;
   const int grid = persistent_blocks(kern, lds, g.ntiles, rt().gram_blocks_per_cu);
@@ -317,15 +372,20 @@ static void gram_block_mfma(T alpha, const T* a, int64_t lda, int na, const T* b
   g.ldb = ldb;
   g.nb = nb;
   const int tm = (na + 15) / 16, tk = (nb + 15) / 16;
+  // paired 16-byte loads: both operands of even width, even leading dimension, aligned
+  const bool pair = tm == 2 && (self || tk == 2) && na % 2 == 0 && nb % 2 == 0 && lda % 2 == 0 &&
+                    ldb % 2 == 0 && ((uintptr_t)a % (2 * sizeof(T))) == 0 && ((uintptr_t)b % (2 * sizeof(T))) == 0;
   int nparts;
   if (self && tm == 1) nparts = gram_mfma_launch<T, 1, 1, true>(g);
+  else if (self && pair) nparts = gram_mfma_launch<T, 2, 2, true, true>(g);
   else if (self) nparts = gram_mfma_launch<T, 2, 2, true>(g);
   else if (tm == 1 && tk == 1) nparts = gram_mfma_launch<T, 1, 1, false>(g);
   else if (tm == 1) nparts = gram_mfma_launch<T, 1, 2, false>(g);
   else if (tk == 1) nparts = gram_mfma_launch<T, 2, 1, false>(g);
+  else if (pair) nparts = gram_mfma_launch<T, 2, 2, false, true>(g);
   else nparts = gram_mfma_launch<T, 2, 2, false>(g);
   const int nout = na * nb;
-  const int fblocks = (nout + 15) / 16;
+  const int fblocks = finalize_blocks(nout);
   hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, na, nb,
                      false, alpha, beta, Cdev, ldc);
   PHX_HIP(hipGetLastError());
@@ -370,7 +430,7 @@ static void gram_block(T alpha, const T* a, int64_t lda, int na, const T* b, int
     // S rows p0.. of this piece land in C rows (or columns, transposed) p0..
     T* cpiece = transposed ? Cdev + (int64_t)p0 * ldc : Cdev + p0;
     const int nout = g.na * nb;
-    const int fblocks = (nout + 15) / 16;
+    const int fblocks = finalize_blocks(nout);
     hipLaunchKernelGGL(k_gram_finalize<T>, dim3(fblocks), dim3(kBlock), 0, rt().stream, g.partials, nparts, g.na, nb,
                        transposed, alpha, beta, cpiece, ldc);
     PHX_HIP(hipGetLastError());
@@ -569,29 +629,44 @@ __global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
       bC[st][tm] = (j < p.k && c < p.m) ? p.alpha * p.cdev[(int64_t)c * p.ldc + j] : T(0);
     }
   const bool vrow = p.vvec && (KS % V16 == 0) && j0 + KS <= p.k;
+  // UNR tiles per iteration where registers allow: all their loads are in
+  // flight before the first MFMA (in place is still safe: the tiles' rows
+  // are disjoint and each row's stores follow its own loads)
+  constexpr int UNR = (GRAM == 0 && KS * TM <= 16) ? 2 : 1;
   const TileWalk tw = xcd_tiles(p.ntiles);
-  for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
-    const int64_t r0 = t * (4 * ROWS) + wv * ROWS;
-    T a[RT][KS];
+  for (int64_t t0 = tw.t; t0 < tw.end; t0 += UNR * tw.stride) {
+  T au[UNR][RT][KS];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const int64_t tu = t0 + u * tw.stride;
+    const int64_t r0 = tu * (4 * ROWS) + wv * ROWS;
+    const bool tok = tu < tw.end;
 #pragma unroll
     for (int rt_ = 0; rt_ < RT; rt_++) {
       const int64_t ra = r0 + rt_ * 16 + cc;
       const T* vp = p.v + ra * p.ldv + j0;
-      if (ra < p.nrows && vrow) {
+      if (tok && ra < p.nrows && vrow) {
 #pragma unroll
-        for (int u = 0; u < KS; u += (KS % V16 == 0 ? V16 : 1)) {
+        for (int q = 0; q < KS; q += (KS % V16 == 0 ? V16 : 1)) {
           if constexpr (KS % V16 == 0) {
             T tmp[V16];
-            vload<T, V16>(vp + u, tmp);
+            vload<T, V16>(vp + q, tmp);
 #pragma unroll
-            for (int e = 0; e < V16; e++) a[rt_][u + e] = tmp[e];
+            for (int e = 0; e < V16; e++) au[u][rt_][q + e] = tmp[e];
           }
         }
       } else {
 #pragma unroll
-        for (int st = 0; st < KS; st++) a[rt_][st] = (ra < p.nrows && j0 + st < p.k) ? vp[st] : T(0);
+        for (int st = 0; st < KS; st++) au[u][rt_][st] = (tok && ra < p.nrows && j0 + st < p.k) ? vp[st] : T(0);
       }
     }
+  }
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const int64_t t = t0 + u * tw.stride;
+    if (t >= tw.end) break;
+    const int64_t r0 = t * (4 * ROWS) + wv * ROWS;
+    T (&a)[RT][KS] = au[u];
     typename M::acc_t acc[RT][TM];
 #pragma unroll
     for (int rt_ = 0; rt_ < RT; rt_++)
@@ -642,6 +717,7 @@ __global__ void __launch_bounds__(kBlock) k_rowmix_mfma(const MixArgs<T> p) {
         }
       }
     }
+  }
   }
   if constexpr (GRAM != 0) {
     // partial tile of this workgroup: S(i, j) at j*na + i
@@ -846,7 +922,7 @@ bool mix_gram(int mode, T alpha, const Mvec<T>* V, const SdMat<T>* C, T beta, Mv
   else if (ks <= 8) nparts = mix_gram_tm<T, 8>(p, !G);
   else nparts = mix_gram_tm<T, 16>(p, !G);
   const int na = G ? G->nvec : m, nout = na * m;
-  hipLaunchKernelGGL(k_gram_finalize<T>, dim3((nout + 15) / 16), dim3(kBlock), 0, rt().stream, p.partials, nparts, na,
+  hipLaunchKernelGGL(k_gram_finalize<T>, dim3(finalize_blocks(nout)), dim3(kBlock), 0, rt().stream, p.partials, nparts, na,
                      m, false, ga, gb, D->dev(), D->root->ld);
   PHX_HIP(hipGetLastError());
   if (to_host) sd_from_device(D);

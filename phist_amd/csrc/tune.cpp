@@ -9,7 +9,10 @@
 //     one, two or four 16-byte vectors per lane);
 //   * per block width, the workgroups per CU of the persistent grid (the
 //     number of waves in flight, which trades latency hiding against the L2
-//     footprint of the gathered x rows).
+//     footprint of the gathered x rows);
+//   * per block width, for rows of at least 16 entries, the depth: 8, 16 or
+//     32 entries of a row whose index loads and gathers go out together (16
+//     and 32 also prefetch the next batch's indices).
 // The choice is stored on the matrix (SparseMat::tuned, used by every later
 // SpMV of that width unless a global setting overrides it) and cached for
 // the process by a matrix signature (scalar size, rows, entries, longest row,
@@ -92,14 +95,15 @@ struct TrialBlock {
   TrialBlock& operator=(const TrialBlock&) = delete;
 };
 
-// fastest of reps products y = A x after one untimed launch; lb/bpc < 0 take
-// the library's usual choice (global settings, tuned entry, table)
+// fastest of reps products y = A x after one untimed launch; lb/bpc/depth < 0
+// take the library's usual choice (global settings, tuned entry, table)
 template <typename T>
-double time_product(const SparseMat<T>* A, Mvec<T>* x, Mvec<T>* y, int lb, int bpc, int reps) {
+double time_product(const SparseMat<T>* A, Mvec<T>* x, Mvec<T>* y, int lb, int bpc, int depth, int reps) {
   Runtime& r = rt();
   SpmvRequest rq;
   rq.lane_bytes = lb;
   rq.blocks_per_cu = bpc;
+  rq.depth = depth;
   spmv_launch<T>(A, T(1), nullptr, x, T(0), y, rq, nullptr, nullptr);
   double best = 1e300;
   for (int k = 0; k < reps; k++) {
@@ -133,16 +137,26 @@ SpmvTuned search_launch(const SparseMat<T>* A, int nvec, int reps) {
   SpmvTuned best;
   best.ms = 1e300;
   std::vector<int> seen;
+  std::vector<int> depths{8};
+  if (A->max_row_len >= 16) depths.push_back(16);
+  if (A->max_row_len >= 24) depths.push_back(32);
   for (int lb : {0, (int)sizeof(T), 32, 64}) {
     const int vpl = lanes_vpl<T>(nvec, lb);
     if (std::find(seen.begin(), seen.end(), vpl) != seen.end()) continue;
     seen.push_back(vpl);
-    for (int bpc : {1, 2, 3, 4, 6, 8}) {
-      const double ms = time_product<T>(A, &x.v, &y.v, lb, bpc, reps);
-      if (ms < best.ms) {
-        best.ms = ms;
-        best.lane_bytes = lb;
-        best.blocks_per_cu = bpc;
+    std::vector<int> tried;
+    for (int d : depths) {
+      const int depth = spmv_effective_depth(d, vpl, (int)sizeof(T), nvec);
+      if (std::find(tried.begin(), tried.end(), depth) != tried.end()) continue;
+      tried.push_back(depth);
+      for (int bpc : {1, 2, 3, 4, 6, 8}) {
+        const double ms = time_product<T>(A, &x.v, &y.v, lb, bpc, depth, reps);
+        if (ms < best.ms) {
+          best.ms = ms;
+          best.lane_bytes = lb;
+          best.blocks_per_cu = bpc;
+          best.depth = depth;
+        }
       }
     }
   }
@@ -151,7 +165,7 @@ SpmvTuned search_launch(const SparseMat<T>* A, int nvec, int reps) {
 
 }  // namespace
 
-// settings[4*i .. 4*i+3] = C, sigma, lane bytes, workgroups per CU for nvecs[i]
+// settings[5*i .. 5*i+4] = C, sigma, lane bytes, workgroups per CU, depth for nvecs[i]
 template <typename T>
 void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, double* default_ms, double* tuned_ms,
                  int* settings) {
@@ -168,7 +182,7 @@ void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, doubl
   A->tuned.clear();
   for (int i = 0; i < n_nvec; i++) {
     TrialBlock<T> x(A->col_map, A->ncols, nvecs[i]), y(A->row_map, A->nrows, nvecs[i]);
-    const double ms = time_product<T>(A, &x.v, &y.v, -1, -1, reps);
+    const double ms = time_product<T>(A, &x.v, &y.v, -1, -1, -1, reps);
     if (default_ms) default_ms[i] = ms;
   }
 
@@ -223,14 +237,15 @@ void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, doubl
   for (int i = 0; i < n_nvec; i++) A->tuned[nvecs[i]] = chosen[i].launch;
   for (int i = 0; i < n_nvec; i++) {
     TrialBlock<T> x(A->col_map, A->ncols, nvecs[i]), y(A->row_map, A->nrows, nvecs[i]);
-    const double ms = time_product<T>(A, &x.v, &y.v, -1, -1, reps);
+    const double ms = time_product<T>(A, &x.v, &y.v, -1, -1, -1, reps);
     A->tuned[nvecs[i]].ms = ms;
     if (tuned_ms) tuned_ms[i] = ms;
     if (settings) {
-      settings[4 * i + 0] = A->C;
-      settings[4 * i + 1] = A->sigma;
-      settings[4 * i + 2] = chosen[i].launch.lane_bytes;
-      settings[4 * i + 3] = chosen[i].launch.blocks_per_cu;
+      settings[5 * i + 0] = A->C;
+      settings[5 * i + 1] = A->sigma;
+      settings[5 * i + 2] = chosen[i].launch.lane_bytes;
+      settings[5 * i + 3] = chosen[i].launch.blocks_per_cu;
+      settings[5 * i + 4] = chosen[i].launch.depth;
     }
   }
   sync_stream();

@@ -21,13 +21,13 @@ def _tune(be, A, nvecs, flags):
     n = len(nvecs)
     nv = np.array(nvecs, dtype=np.int32)
     dms, tms = np.zeros(n), np.zeros(n)
-    st = np.zeros(4 * n, dtype=np.int32)
+    st = np.zeros(5 * n, dtype=np.int32)
     fl = ctypes.c_int(0)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     getattr(be.L, f"phist_{be.p}amd_sparseMat_tune")(A.h, ctypes.c_int(n), p(nv), ctypes.c_int(flags), p(dms),
                                                       p(tms), p(st), ctypes.byref(fl))
     assert fl.value == 0
-    return dms, tms, st.reshape(n, 4)
+    return dms, tms, st.reshape(n, 5)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -49,9 +49,9 @@ def test_tune_layout_and_launch(be64, be32, dtype):
         y.delete()
     dms, tms, st = _tune(be, A, [1, 8], 1 | 2)
     item = np.dtype(dtype).itemsize
-    for (C, sigma, lb, bpc), d, t in zip(st, dms, tms):
+    for (C, sigma, lb, bpc, depth), d, t in zip(st, dms, tms):
         assert C in (16, 32, 64) and sigma in (1, 256)
-        assert lb in (0, item, 32, 64) and bpc in (1, 2, 3, 4, 6, 8)
+        assert lb in (0, item, 32, 64) and bpc in (1, 2, 3, 4, 6, 8) and depth in (8, 16, 32)
         assert 0 < t <= 1.5 * d + 0.02, (t, d)
     s = A.sell()
     assert s["C"] == st[0][0]
@@ -92,3 +92,39 @@ def test_tune_fp32_twin(be64):
     for o in (x, y, A):
         o.delete()
     m.delete()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [1, 4, 8, 16])
+def test_spmv_depth_bitexact(be64, be32, dtype, nvec):
+    """depths 16 and 32 (the next batch's indices prefetched; a whole 32-entry
+    row's gathers in flight) and depth 8 give the same product bit for bit, on rows of 32 entries, of 40 entries (the
+    second batch of a row mostly empty) and of at most 7 (a 3D Laplacian:
+    every batch mostly empty), and agree with the oracle"""
+    be = be64 if dtype == np.float64 else be32
+    fl = ctypes.c_int(0)
+    for n, nnz, hb in ((50000, 32, 3000), (30000, 40, 500), (8000, 0, 0)):
+        A = be.banded_random(n, nnz, hb, 40.0, 17 + nnz) if nnz else be.laplace3d(20, 20, 20)
+        ro, ci, va = A.crs()
+        ref = ol.Crs(n, n, ro, ci, va.astype(float))
+        m = be.map(n)
+        X = np.random.default_rng(nvec).uniform(-1, 1, (n, nvec)).astype(dtype)
+        x = be.mvec(m, nvec, X)
+        out = {}
+        try:
+            for depth in (8, 16, 32):
+                be.L.phist_amd_set_spmv_depth(depth, ctypes.byref(fl))
+                assert fl.value == 0
+                y = be.mvec(m, nvec)
+                be.spmv(1.0, A, x, 0.0, y)
+                out[depth] = y.get()
+                y.delete()
+        finally:
+            be.L.phist_amd_set_spmv_depth(0, ctypes.byref(fl))
+        assert np.array_equal(out[8], out[16]) and np.array_equal(out[8], out[32])
+        want = ol.spmv(ref, 1.0, X.astype(float), 0.0, np.zeros((n, nvec)))
+        num = np.linalg.norm(out[32].astype(float) - want, axis=0)
+        assert np.all(num <= (1e-12 if dtype == np.float64 else 1e-5) * np.linalg.norm(want, axis=0))
+        x.delete()
+        A.delete()
+        m.delete()
