@@ -175,9 +175,13 @@ void phist_amd_jada_set_inner_fp32(int enable, int* iflag);
    together by blocked MINRES (A symmetric), starting from x = 0, until
    ||b_j - (A + shifts[j]) x_j|| <= tol[j] ||b_j|| (the MINRES estimate) or
    maxIter iterations; converged systems stop updating.  With fp32 != 0 the
-   iteration runs in FP32 on a cached FP32 copy of A (B and X stay FP64).
-   nIter[j] and relres[j] (estimates) are returned per system; iflag > 0 is
-   the number of systems that did not converge. */
+   iteration runs in FP32 on a cached FP32 copy of A (B and X stay FP64) and
+   is refined in FP64: between FP32 solves the true residuals are formed in
+   FP64 and the systems still above tol solve for their correction again
+   (at most maxIter iterations in all); relres[j] is then the true FP64
+   residual, nIter[j] the iterations of all rounds.  In FP64, nIter[j] and
+   relres[j] (the MINRES estimate) are returned per system; iflag > 0 is the
+   number of systems that did not converge. */
 void phist_amd_DblockedMINRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, const double shifts[],
                                       phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[], int maxIter,
                                       int fp32, int nIter[], double relres[], int* iflag);

@@ -142,7 +142,10 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
     }
 
     // ---- one GMRES(m) cycle on the kk systems: v_0 = r / ||r|| ----
-    MvBox<TI> w(map, kk), Vb(map, (m + 1) * kk);
+    // vcur: the current basis vectors as a compact block (row stride kk), the
+    // SpMV's input; gathering them from the side-by-side basis (row stride
+    // (m+1)*kk) would fetch a cache line per entry for a few bytes of it
+    MvBox<TI> w(map, kk), Vb(map, (m + 1) * kk), vcur(map, kk);
     std::vector<double> H((size_t)kk * (m + 1) * m), cs((size_t)kk * m), sn((size_t)kk * m), g((size_t)kk * (m + 1));
     auto h = [&](int a, int r, int c) -> double& { return H[((size_t)a * m + c) * (m + 1) + r]; };
     std::vector<TI> sc(kk);
@@ -152,7 +155,8 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
     }
     {
       View<TI> v0(Vb.get(), 0, kk - 1);
-      blas_axpby<TI>(sc.data(), false, wc->get(), TI(0), v0.get());
+      blas_axpby<TI>(sc.data(), false, wc->get(), TI(0), vcur.get());
+      blas_copy<TI>(vcur.get(), v0.get());
     }
     std::vector<int> active(kk, 1), steps(kk, 0);
     // per-system sweeps (each system meets only its own basis): device work for the partial sums, the
@@ -185,9 +189,8 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
       bool live = false;
       for (int a = 0; a < kk; a++) live = live || active[a];
       if (!live) break;
-      View<TI> vi(Vb.get(), i * kk, (i + 1) * kk - 1);
       View<TI> Vi(Vb.get(), 0, (i + 1) * kk - 1);
-      op.apply_shifted(TI(1), s.data(), vi.get(), TI(0), w.get(), nullptr);
+      op.apply_shifted(TI(1), s.data(), vcur.get(), TI(0), w.get(), nullptr);
       project(w.get());
       std::vector<double> hc((size_t)(i + 1) * kk, 0.0);
       std::vector<TI> d(kk), inv(kk);
@@ -276,12 +279,13 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
         }
       }
       View<TI> vn(Vb.get(), (i + 1) * kk, (i + 2) * kk - 1);
-      blas_axpby<TI>(inv.data(), false, w.get(), TI(0), vn.get());
+      blas_axpby<TI>(inv.data(), false, w.get(), TI(0), vcur.get());
       for (int a = 0; a < kk; a++)  // a non-finite column must not reach the other systems through 0 * NaN
         if (!std::isfinite((double)d[a])) {
-          View<TI> col(vn.get(), a, a);
+          View<TI> col(vcur.get(), a, a);
           blas_put_value<TI>(col.get(), TI(0));
         }
+      blas_copy<TI>(vcur.get(), vn.get());
     }
     // ---- x += V y per system (y from the triangular systems R y = g) ----
     int L = 0;

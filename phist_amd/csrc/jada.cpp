@@ -809,6 +809,84 @@ void phist_amd_jada_set_inner_fp32(int enable, int* iflag) {
   g_inner_fp32 = enable != 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// FP32 iterations refined in FP64 (mixed-precision iterative refinement):
+// after an FP32 solve the true residuals r_j = b_j - (A + s_j) x_j are
+// formed in FP64, and the systems still above tol[j] solve
+// (A + s_j) d_j = r_j again in FP32, x_j += d_j.  An FP32 recurrence's own
+// residual estimate parts from the true residual once it reaches the FP32
+// attainable accuracy (about eps32 times the condition number of the shifted
+// system), so an FP32 solve alone can report convergence far above tol.
+// A round asks the FP32 solve for the reduction still needed, but not below
+// kRefineInnerTol; refinement stops when every system is below tol, the
+// iteration budget is spent, or a round improves no unconverged system by
+// at least 10 %.  iters sums the rounds; relres is the true FP64 residual.
+constexpr double kRefineInnerTol = 1e-3;
+constexpr int kRefineRounds = 30;
+template <class Solve>
+MinresOut<double> fp32_refined(const SparseMat<double>* M, const std::vector<double>& s, const Mvec<double>* b,
+                               Mvec<double>* x, const std::vector<double>& tol, int maxIter, Solve&& solve) {
+  const int k = b->nvec;
+  MinresOut<double> out;
+  out.iters.assign(k, 0);
+  out.relres.assign(k, 0.0);
+  {
+    const auto r0 = solve(b, x, tol, maxIter);
+    for (int j = 0; j < k; j++) out.iters[j] = r0.iters[j];
+  }
+  Mvec<double>* R = api::mvec_create<double>(b->map, k);
+  Mvec<double>* D = api::mvec_create<double>(b->map, k);
+  try {
+    std::vector<double> nb(k), nr(k), prev(k, 1e300), tin(k);
+    blas_dot<double>(b, b, nb.data());
+    for (auto& v : nb) v = std::sqrt(v);
+    SpmvRequest rq;
+    rq.has_shift = true;
+    rq.has_beta = true;
+    for (int round = 1;; round++) {
+      blas_copy<double>(b, R);
+      spmv_launch<double>(M, -1.0, s.data(), x, 1.0, R, rq, nullptr, nullptr);
+      blas_dot<double>(R, R, nr.data());
+      int used = 0;
+      bool more = false, progress = false;
+      for (int j = 0; j < k; j++) {
+        out.relres[j] = nb[j] > 0.0 ? std::sqrt(nr[j]) / nb[j] : 0.0;
+        const bool conv = out.relres[j] <= tol[j];
+        if (!conv && out.iters[j] < maxIter) {
+          more = true;
+          used = std::max(used, out.iters[j]);
+          if (out.relres[j] < 0.9 * prev[j]) progress = true;
+        }
+        prev[j] = out.relres[j];
+        tin[j] = conv ? 2.0 : std::min(0.5, std::max(kRefineInnerTol, tol[j] / out.relres[j]));
+      }
+      if (!more || round > kRefineRounds || (round > 1 && !progress)) break;
+      blas_put_value<double>(D, 0.0);
+      const auto r = solve(R, D, tin, maxIter - used);
+      for (int j = 0; j < k; j++)
+        if (out.relres[j] > tol[j]) out.iters[j] += r.iters[j];
+      const double one = 1.0;
+      blas_axpby<double>(&one, true, D, 1.0, x);
+    }
+  } catch (...) {
+    api::mvec_delete<double>(R);
+    api::mvec_delete<double>(D);
+    throw;
+  }
+  api::mvec_delete<double>(R);
+  api::mvec_delete<double>(D);
+  out.unconverged = 0;
+  for (int j = 0; j < k; j++) out.unconverged += out.relres[j] > tol[j] ? 1 : 0;
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
 void phist_amd_DblockedMINRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, const double shifts[],
                                       phist_Dconst_mvec_ptr B, phist_Dmvec_ptr X, const double tol[], int maxIter,
                                       int fp32, int nIter[], double relres[], int* iflag) {
@@ -827,7 +905,11 @@ void phist_amd_DblockedMINRES_shifted(phist_Dconst_sparseMat_ptr A, int nsys, co
     if (fp32) {
       Operator<float> op;
       op.A = sparse_fp32_twin(M);
-      MinresOut<float> r = minres_solve<double, float>(op, s, nullptr, b, x, tl, maxIter);
+      const MinresOut<double> r =
+          fp32_refined(M, s, b, x, tl, maxIter, [&](const Mvec<double>* rhs, Mvec<double>* sol,
+                                                   const std::vector<double>& t, int budget) {
+            return minres_solve<double, float>(op, s, nullptr, rhs, sol, t, budget);
+          });
       for (int j = 0; j < nsys; j++) {
         nIter[j] = r.iters[j];
         relres[j] = r.relres[j];
@@ -874,7 +956,10 @@ void phist_amd_DblockedMINRES_shifted_refill(phist_Dconst_sparseMat_ptr A, int n
     if (fp32) {
       Operator<float> op;
       op.A = sparse_fp32_twin(M);
-      put(minres_refill_solve<double, float>(op, s, b, x, tl, maxIter, std::min(slots, nsys)));
+      put(fp32_refined(M, s, b, x, tl, maxIter, [&](const Mvec<double>* rhs, Mvec<double>* sol,
+                                                   const std::vector<double>& t, int budget) {
+        return minres_refill_solve<double, float>(op, s, rhs, sol, t, budget, std::min(slots, nsys));
+      }));
     } else {
       Operator<double> op;
       op.A = M;
