@@ -232,6 +232,22 @@ def jada_bench(be, args):
         name = f"3D Laplacian {n}^3 ({A.nrows} rows), 8 smallest"
     t_build = time.perf_counter() - t0
     print(f"bench: {name}; built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_tune:
+        # set-up, not timed: the library's autotuner picks the SpMV launch for the block width on the
+        # matrix and on its FP32 copy (the correction solves of the mixed run)
+        pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        nv, dms, tms, st = (np.array([cfg["block_size"]], np.int32), np.zeros(1), np.zeros(1),
+                            np.zeros(5, np.int32))
+        iflag = ctypes.c_int(0)
+        t0 = time.perf_counter()
+        be.L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 4, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
+        if iflag.value:
+            raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
+        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]),
+                "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
+                "tune_s": round(time.perf_counter() - t0, 2)}
+        print(f"bench: autotuner (FP64 matrix; its FP32 copy tuned too) {tune}", file=sys.stderr, flush=True)
     # untimed warm-up (three outer iterations): the solver's blocks come from the
     # allocation caches in both timed runs, not only in the second
     cfg["inner"] = args.inner
@@ -278,7 +294,8 @@ def jada_bench(be, args):
     out = {"metric": "JD time-to-1e-8 mixed vs FP64", "value": main["seconds"], "unit": "s",
            "n_gpus": 1, "higher_is_better": False, "dtype": "f64 outer / f32 inner",
            "data": "synthetic (generated matrix, KISS random start vectors)",
-           "config": {"workload": name, "which": "SR", "tol": 1e-8, **cfg},
+           "config": {"workload": name, "which": "SR", "tol": 1e-8, **cfg,
+                      **({} if tune is None else {"autotuned": tune})},
            **results, "speedup_mixed_vs_fp64": None if speed is None else round(speed, 3)}
     print(json.dumps(out), flush=True)
     A.delete()
@@ -741,7 +758,7 @@ def main():
     ap.add_argument("--depth", type=int, default=0, choices=[0, 8, 16, 32],
                     help="--op spmv: entries per row whose loads the SpMV issues together (0: tuned / default)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="--op spmv: skip the autotuner (launch settings from the library's table)")
+                    help="--op spmv / jada: skip the autotuner (launch settings from the library's table)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -758,6 +775,10 @@ def main():
         L.phist_bench_stream_copy(ctypes.byref(mean_bw), ctypes.byref(max_bw), ctypes.byref(iflag))
         print(f"bench: STREAM copy {mean_bw.value:.0f} GB/s mean, {max_bw.value:.0f} GB/s max",
               file=sys.stderr, flush=True)
+        for kind, label in ((0, "read"), (1, "copy")):
+            L.phist_amd_bench_hbm(kind, ctypes.byref(mean_bw), ctypes.byref(max_bw), ctypes.byref(iflag))
+            print(f"bench: HBM ceiling {label} (16-byte accesses, four in flight) {mean_bw.value:.0f} GB/s mean, "
+                  f"{max_bw.value:.0f} GB/s max", file=sys.stderr, flush=True)
     if args.gather_calibration:
         mean_bw, max_bw = ctypes.c_double(), ctypes.c_double()
         L.phist_amd_bench_gather(args.gather_calibration, ctypes.byref(mean_bw), ctypes.byref(max_bw),

@@ -96,6 +96,12 @@ void phist_amd_kernel_timer_read(double* total_ms, int64_t* count, int* iflag);
  * load per lane; mean and best GB/s of 20 launches.  A counter profile of it
  * calibrates the L2 fetch counters for gathers of that size. */
 void phist_amd_bench_gather(int seg_bytes, double* mean_bw, double* max_bw, int* iflag);
+/* the device's streaming ceiling as this library's kernels can reach it:
+ * kind 0 reads, kind 1 copies a 1 GiB array with 16-byte accesses, four in
+ * flight per thread, each workgroup over a contiguous part; mean and best GB/s of 20
+ * launches (bytes read + written).  phist_bench_stream_* are the reference's
+ * plain STREAM kernels (one 8-byte element per thread and step). */
+void phist_amd_bench_hbm(int kind, double* mean_bw, double* max_bw, int* iflag);
 
 /* phist_Dfused_spmv_mvdot that allocates nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under

@@ -364,6 +364,75 @@ __global__ void __launch_bounds__(256) k_gather_once(const double* __restrict__ 
   if (s0 + s1 == 12345.678) out[0] = s0;  // keeps the loads alive
 }
 
+// Device-ceiling kernels: 16-byte accesses, four independent ones in flight
+// per thread; each workgroup walks its own contiguous part of the array in
+// steps of 4 x 256 vectors (16 KB), the form a bandwidth-bound kernel of this
+// library can at best take.  kind 0 reads n doubles, kind 1 copies them.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_hbm_ceiling(const double2* __restrict__ a, double2* __restrict__ b, int64_t nv,
+                                                     double* out) {
+  const int64_t per = (nv + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = blockIdx.x * per, hi = lo + per < nv ? lo + per : nv;
+  double s = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 4 * 256) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = i + u * 256;
+      v[u] = j < hi ? a[j] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = i + u * 256;
+      if (KIND == 0) {
+        s += v[u].x + v[u].y;
+      } else if (j < hi) {
+        b[j] = v[u];
+      }
+    }
+  }
+  if (KIND == 0 && s == 12345.678) out[0] = s;  // keeps the loads alive
+}
+
+// kind 0 read / 1 copy of 1 GiB with k_hbm_ceiling; mean/max GB/s over reps
+void run_ceiling(int kind, double* mean_bw, double* max_bw) {
+  ensure_init();
+  Runtime& r = rt();
+  PHX_REQUIRE(kind == 0 || kind == 1, PHIST_INVALID_INPUT, "kind %d: 0 (read) or 1 (copy)", kind);
+  const int64_t n = int64_t(1) << 27;  // 1 GiB
+  double *a = nullptr, *b = nullptr, *c = nullptr;
+  PHX_HIP(hipMalloc(&a, n * 8));
+  PHX_HIP(hipMalloc(&b, n * 8));
+  PHX_HIP(hipMalloc(&c, 64));
+  PHX_HIP(hipMemsetAsync(a, 0, n * 8, r.stream));
+  const int blocks = r.num_cu * 8;
+  const int reps = 20;
+  double sum = 0, best = 0;
+  for (int it = -2; it < reps; it++) {
+    PHX_HIP(hipEventRecord(r.ev_start, r.stream));
+    if (kind == 0)
+      hipLaunchKernelGGL(k_hbm_ceiling<0>, dim3(blocks), dim3(256), 0, r.stream, (const double2*)a, (double2*)b,
+                         n / 2, c);
+    else
+      hipLaunchKernelGGL(k_hbm_ceiling<1>, dim3(blocks), dim3(256), 0, r.stream, (const double2*)a, (double2*)b,
+                         n / 2, c);
+    PHX_HIP(hipGetLastError());
+    PHX_HIP(hipEventRecord(r.ev_stop, r.stream));
+    PHX_HIP(hipEventSynchronize(r.ev_stop));
+    float ms = 0;
+    PHX_HIP(hipEventElapsedTime(&ms, r.ev_start, r.ev_stop));
+    if (it < 0) continue;
+    const double gbs = (kind == 0 ? 8.0 : 16.0) * n / (ms * 1e-3) / 1e9;
+    sum += gbs;
+    best = std::max(best, gbs);
+  }
+  PHX_HIP(hipFree(a));
+  PHX_HIP(hipFree(b));
+  PHX_HIP(hipFree(c));
+  *mean_bw = sum / reps;
+  *max_bw = best;
+}
+
 // one launch of k_gather_once over 1 GiB with seg_bytes-byte segments
 // (16..256), timed over reps after two untimed launches
 void run_gather(int seg_bytes, double* mean_bw, double* max_bw) {
@@ -615,6 +684,9 @@ void phist_bench_stream_triad(double* mean_bw, double* max_bw, int* iflag) {
 }
 void phist_amd_bench_gather(int seg_bytes, double* mean_bw, double* max_bw, int* iflag) {
   guard(iflag, __func__, [&] { run_gather(seg_bytes, mean_bw, max_bw); *iflag = 0; });
+}
+void phist_amd_bench_hbm(int kind, double* mean_bw, double* max_bw, int* iflag) {
+  guard(iflag, __func__, [&] { run_ceiling(kind, mean_bw, max_bw); *iflag = 0; });
 }
 
 // ---- extension hooks (phist_amd_ext.h) ----

@@ -268,3 +268,31 @@ def test_jada_laplace_config1_shape(be64, fp32, inner):
     check_schur(be64, A, Q, R, 8, 1e-8)
     for o in (Q, R, A):
         o.delete()
+
+
+def test_jada_xxz24_full_size_both_precisions(be64):
+    """the mixed-precision eigensolver workload at full size: Heisenberg chain L=24, S_z = 0
+    (2,704,156 rows), 16 smallest eigenpairs to 1e-8 with blocked-GMRES correction solves (at most
+    10 iterations, the reference's default solver), FP64 throughout and with FP32 correction
+    solves.  Both runs: 16 converged, residuals <= tol, the partial Schur form's invariants
+    (||A Q - Q R|| <= 10 tol per column, Q'Q = I) checked with the oracle's product; the mixed run's
+    eigenvalues equal the FP64 run's to within the tolerance (a residual of tol moves a symmetric
+    eigenvalue by at most tol * ||A||-scale, far less here)."""
+    A = be64.spinchain_sz(24, 12)
+    assert A.nrows == 2704156
+    tol, k = 1e-8, 16
+    evs = {}
+    for fp32 in (False, True):
+        ev, res, nconv, niter, Q, R = be64.subspacejada(A, k, "SR", tol, block_size=4, min_bas=24, max_bas=48,
+                                                        max_iters=300, inner="GMRES", inner_max_iters=10,
+                                                        inner_fp32=fp32)
+        print(f"L=24 fp32={fp32}: {niter} iterations, nconv {nconv}, ev[:3] {ev[:3]}", flush=True)
+        assert nconv >= k, (nconv, niter, res)
+        assert np.all(res[:k] <= tol), res[:k]
+        check_schur(be64, A, Q, R, k, tol)
+        evs[fp32] = np.sort(ev[:k])
+        for o in (Q, R):
+            o.delete()
+    scale = np.maximum(1.0, np.abs(evs[False]))
+    assert np.all(np.abs(evs[True] - evs[False]) <= 10 * tol * scale), (evs[True], evs[False])
+    A.delete()
