@@ -295,6 +295,8 @@ struct SparseMat {
   std::vector<int64_t> crs_ptr;
   std::vector<int32_t> crs_col;
   std::vector<T> crs_val;
+  // A == A' entry by entry (-1: not yet checked; sparse_is_symmetric fills it)
+  mutable int symmetric = -1;
   // SELL-C-sigma on the device
   int C = 0, sigma = 0;
   int64_t nchunks = 0;

@@ -652,14 +652,57 @@ void subspacejada(const Operator<T>& A, const Operator<float>* A32, const phist_
 
 }  // namespace
 
+// A == A' entry by entry for a square matrix of this library (the sums of
+// duplicate entries compared, rows of the host CRS sorted by column); the
+// answer is kept on the matrix
+template <typename T>
+bool sparse_is_symmetric(const SparseMat<T>* A) {
+  if (A->symmetric >= 0) return A->symmetric == 1;
+  bool sym = A->nrows == A->ncols;
+  const int64_t n = A->nrows;
+  const int64_t* ptr = A->crs_ptr.data();
+  const int32_t* col = A->crs_col.data();
+  const T* val = A->crs_val.data();
+  // sum of the entries (r, c) of row r (0 when absent): binary search in the sorted row
+  auto entry = [&](int64_t r, int32_t c) {
+    const int32_t* lo = std::lower_bound(col + ptr[r], col + ptr[r + 1], c);
+    T s = T(0);
+    for (const int32_t* p = lo; p < col + ptr[r + 1] && *p == c; p++) s += val[p - col];
+    return s;
+  };
+  if (sym) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad)
+    for (int64_t i = 0; i < n; i++) {
+      for (int64_t p = ptr[i]; p < ptr[i + 1] && !bad; p++) {
+        if (p > ptr[i] && col[p] == col[p - 1]) continue;  // a duplicate's sum was compared with its first entry
+        if (!(entry(i, col[p]) == entry(col[p], (int32_t)i))) bad = 1;
+      }
+    }
+    sym = bad == 0;
+  }
+  A->symmetric = sym ? 1 : 0;
+  return sym;
+}
+template bool sparse_is_symmetric<double>(const SparseMat<double>*);
+template bool sparse_is_symmetric<float>(const SparseMat<float>*);
+
 // The option checks of phist_Xsubspacejada, shared by the direct call and
 // the service loop: every combination outside the symmetric standard
 // problem with MINRES, GMRES or no correction solver returns -99
-// (PHIST_NOT_IMPLEMENTED) instead of running something else.
-void jada_validate(const phist_jadaOpts& opts) {
+// (PHIST_NOT_IMPLEMENTED) instead of running something else.  The
+// reference's default symmetry (phist_GENERAL, phist_jadaOpts.cpp:21) is
+// accepted when the operator is a matrix of this library that is symmetric
+// entry by entry (matrix_symmetric): the symmetric algorithm then computes
+// the same partial Schur form the general one would (R diagonal up to the
+// couplings of locked pairs, Q orthonormal).
+void jada_validate(const phist_jadaOpts& opts, bool matrix_symmetric) {
   PHX_REQUIRE(opts.how == phist_STANDARD, PHIST_NOT_IMPLEMENTED, "only standard Ritz extraction");
-  PHX_REQUIRE(opts.symmetry == phist_HERMITIAN || opts.symmetry == phist_COMPLEX_SYMMETRIC, PHIST_NOT_IMPLEMENTED,
-              "only symmetric problems (opts.symmetry HERMITIAN)");
+  PHX_REQUIRE(opts.symmetry == phist_HERMITIAN || opts.symmetry == phist_COMPLEX_SYMMETRIC ||
+                  (opts.symmetry == phist_GENERAL && matrix_symmetric),
+              PHIST_NOT_IMPLEMENTED,
+              "only symmetric problems (opts.symmetry HERMITIAN, or GENERAL on a matrix of this library that is "
+              "symmetric)");
   PHX_REQUIRE(opts.which == phist_LM || opts.which == phist_SM || opts.which == phist_LR || opts.which == phist_SR,
               PHIST_NOT_IMPLEMENTED, "eigenvalue selection %d not supported", (int)opts.which);
   PHX_REQUIRE(opts.innerSolvType == phist_MINRES || opts.innerSolvType == phist_GMRES ||
@@ -692,10 +735,10 @@ void run_jada(const typename LinOpT<T>::type* AB_op, const void* B_op, const phi
               T* ev, T* resNorm, int* nConv, int* nIter) {
   PHX_REQUIRE(AB_op != nullptr, PHIST_INVALID_INPUT, "no operator");
   PHX_REQUIRE(B_op == nullptr, PHIST_NOT_IMPLEMENTED, "generalized problems (B_op) are not supported");
-  jada_validate(opts);
   Operator<T> op;
   op.A = own_matrix<T>(AB_op);
   if (!op.A) op.user = AB_op;
+  jada_validate(opts, op.A && opts.symmetry == phist_GENERAL && sparse_is_symmetric(op.A));
   Operator<float> op32;
   const Operator<float>* lowered = nullptr;
   if constexpr (std::is_same<T, double>::value) {
@@ -718,7 +761,7 @@ struct JadaHandle {
 
 JadaHandle* jada_start(const SparseMat<double>* A, bool fp32, const phist_jadaOpts& opts, Mvec<double>* Q,
                        SdMat<double>* R, double* ev, double* resNorm) {
-  jada_validate(opts);
+  jada_validate(opts, opts.symmetry == phist_GENERAL && sparse_is_symmetric(A));
   std::unique_ptr<JadaHandle> h(new JadaHandle());
   h->op.A = A;
   const Operator<float>* lowered = nullptr;

@@ -107,7 +107,8 @@ class Server {
   int64_t submit_jada(const SparseMat<double>* A, const phist_jadaOpts& opts, bool fp32, Mvec<double>* Q,
                       SdMat<double>* R, double* ev, double* res) {
     PHX_REQUIRE(A && Q && R && ev && res, PHIST_INVALID_INPUT, "bad eigenvalue request");
-    jada_validate(opts);  // unsupported options fail at submission, as phist_Dsubspacejada fails
+    // unsupported options fail at submission, as phist_Dsubspacejada fails
+    jada_validate(opts, opts.symmetry == phist_GENERAL && sparse_is_symmetric(A));
     auto r = std::make_unique<Request>();
     r->id = ++last_id_;
     r->jada = true;

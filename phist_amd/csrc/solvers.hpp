@@ -849,8 +849,11 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
 // (defined in jada.cpp): start builds the start-up basis, step runs one outer
 // iteration (true when finished), finish writes the outputs
 struct JadaHandle;
-// throws -99 for options outside what subspacejada supports, -55 for inconsistent sizes
-void jada_validate(const phist_jadaOpts& opts);
+// throws -99 for options outside what subspacejada supports, -55 for inconsistent sizes;
+// matrix_symmetric: the operator is a matrix of this library, symmetric entry by entry
+// (then the reference's default symmetry, phist_GENERAL, is accepted)
+void jada_validate(const phist_jadaOpts& opts, bool matrix_symmetric);
+template <typename T> bool sparse_is_symmetric(const SparseMat<T>* A);
 JadaHandle* jada_start(const SparseMat<double>* A, bool fp32, const phist_jadaOpts& opts, Mvec<double>* Q,
                        SdMat<double>* R, double* ev, double* resNorm);
 bool jada_step(JadaHandle* h);

@@ -12,6 +12,7 @@ GMRES, phist_jadaCorrectionSolver_def.hpp:499-512; GMRES is the reference's
 default, phist_jadaOpts.cpp:40).
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -189,25 +190,44 @@ def test_jada_integration_config_spinSZ14(be64, fp32):
         o.delete()
 
 
-def test_jada_rejects_unsupported(be64):
+def _run_defaults(be64, A, nq):
+    """phist_Dsubspacejada with the reference's default options (phist_jadaOpts_setDefaults)"""
     import ctypes
     import phist_amd as pa
-    A = be64.laplace3d(4, 4, 4)
     opts = pa.JadaOpts()
     be64.L.phist_jadaOpts_setDefaults(ctypes.byref(opts))
-    assert opts.numEigs == 6 and opts.maxBas == 20 and abs(opts.convTol - 1e-12) < 1e-20  # reference defaults
     op = pa.LinearOp()
     fl = ctypes.c_int(0)
     be64.L.phist_DlinearOp_wrap_sparseMat(ctypes.byref(op), A.h, ctypes.byref(fl))
-    Q, R = be64.mvec(A.domain_map, 6), be64.sdmat(6, 6)
-    ev, res = np.zeros(6), np.zeros(6)
+    Q, R = be64.mvec(A.domain_map, nq), be64.sdmat(nq, nq)
+    ev, res = np.zeros(nq), np.zeros(nq)
     nc, ni = ctypes.c_int(0), ctypes.c_int(0)
-    # symmetry GENERAL (the default) is not implemented: -99, no work done
     be64.L.phist_Dsubspacejada(ctypes.byref(op), None, opts, Q.h, R.h, ev.ctypes.data_as(ctypes.c_void_p),
                                res.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nc), ctypes.byref(ni),
                                ctypes.byref(fl))
-    assert fl.value == -99
+    return opts, fl.value, ev, res, nc.value, Q, R
+
+
+def test_jada_reference_defaults(be64):
+    """The reference's defaults (phist_jadaOpts.cpp: 6 largest-magnitude eigenpairs, symmetry GENERAL,
+    GMRES correction solves, tol 1e-12, maxBas 20): on a matrix of this library that is symmetric entry by entry the
+    symmetric algorithm runs and finds the closed-form Laplacian eigenvalues; on a nonsymmetric matrix
+    the general (Schur) algorithm is not implemented: -99, no work done."""
+    import ctypes
+    import phist_amd as pa
+    A = be64.laplace3d(5, 6, 7)
+    opts, fl, ev, res, nc, Q, R = _run_defaults(be64, A, 6)
+    assert opts.numEigs == 6 and opts.maxBas == 20 and abs(opts.convTol - 1e-12) < 1e-20  # reference defaults
+    assert opts.symmetry != pa.HERMITIAN and opts.innerSolvType == pa.LINSOLV["GMRES"]
+    assert fl == 0 and nc >= 6, (fl, nc, res)
+    want = laplace_eigs(5, 6, 7, 5 * 6 * 7)[-6:]  # which = LM (the default): the six largest
+    assert np.allclose(np.sort(ev[:6]), want, rtol=0, atol=1e-10), (np.sort(ev[:6]), want)
     for o in (Q, R, A):
+        o.delete()
+    B = be64.read_mm(os.path.join(ol.MATRICES, "Dsprandn25.mm"))  # nonsymmetric
+    _, fl, _, _, _, Q, R = _run_defaults(be64, B, 6)
+    assert fl == -99
+    for o in (Q, R, B):
         o.delete()
 
 
