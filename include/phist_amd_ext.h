@@ -39,8 +39,8 @@ void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag);
  * together before their multiply-adds (0 = the default: the matrix's tuned
  * value, else 8; 8, 16 or 32).  16 and 32 also request the next batch's
  * val/col before the current batch's gathers; they apply to row groups of at
- * most 8 lanes gathering at most 32 (depth 16) or 16 bytes (depth 32) each,
- * without per-column x factors, and fall back to 8 otherwise.  Every choice
+ * most 8 lanes gathering at most 32 bytes each, without per-column x
+ * factors, and fall back to 8 otherwise.  Every choice
  * gives the same result bit for bit. */
 void phist_amd_set_spmv_depth(int slots, int* iflag);
 /* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16

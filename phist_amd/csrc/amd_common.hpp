@@ -262,16 +262,13 @@ template <typename T> void sd_from_device_async(const SdMat<T>* M);
 // lane gathers per nonzero (0 = one 16-byte vector) and workgroups per CU
 // (0 = as many as fit)
 // the depth the SpMV runs for a requested depth (8, 16 or 32), the scalars
-// per lane vpl, the scalar size and the block width: 32 needs at most 16
-// bytes per lane, 16 at most 32, and both at most 8 lanes per row (blocks of
-// at most 64 columns' lane groups); otherwise the next smaller depth
+// per lane vpl, the scalar size and the block width: 16 and 32 need at most
+// 32 gathered bytes per lane and at most 8 lanes per row; otherwise 8
 inline int spmv_effective_depth(int depth, int vpl, int scalar_bytes, int ncols) {
   int groups = (std::min(ncols, 64 * vpl) + vpl - 1) / vpl, lpr = 1;
   while (lpr < groups) lpr <<= 1;
-  if (lpr > 8) return 8;
-  if (depth >= 32 && vpl * scalar_bytes <= 16) return 32;
-  if (depth >= 16 && vpl * scalar_bytes <= 32) return 16;
-  return 8;
+  if (lpr > 8 || vpl * scalar_bytes > 32) return 8;
+  return depth >= 32 ? 32 : (depth >= 16 ? 16 : 8);
 }
 
 struct SpmvTuned {

@@ -146,7 +146,9 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
   constexpr int RPB = (kBlock / 64) * RPW;
   constexpr int KB = SL > LPR ? SL / LPR : 1;  // val/col loads per lane and batch
   constexpr int BATCH = KB * LPR;              // slots per batch and row
-  constexpr int SUBMAX = SL >= 32 ? 32 : 16;
+  // gathers issued before their FMAs: up to 32 at depth 32 with at most 16
+  // gathered bytes per lane, otherwise 16 (register budget)
+  constexpr int SUBMAX = (SL >= 32 && VPL * (int)sizeof(T) <= 16) ? 32 : 16;
   constexpr int SUB = BATCH < SUBMAX ? BATCH : SUBMAX;  // gathers issued before their FMAs
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -476,13 +478,13 @@ int launch_deep(SpmvArgs<T>& a, int lpr) {
   }
 }
 
-// the deeper batches hold depth * VPL gathered values per lane: depth 32
-// takes at most one 16-byte vector per lane, depth 16 at most two (wider
-// lanes would need more registers than the file holds without spilling)
+// the deeper batches hold their gathered values in registers: depth 16 and
+// 32 take at most two 16-byte vectors per lane (wider lanes would need more
+// registers than the file holds without spilling)
 template <typename T, int VPL>
 int launch_xsc(SpmvArgs<T>& a, int lpr, int depth) {
   if (!a.xscale && lpr <= 8) {
-    if constexpr (VPL * sizeof(T) <= 16)
+    if constexpr (VPL * sizeof(T) <= 32)
       if (depth >= 32) return launch_deep<T, VPL, 32>(a, lpr);
     if constexpr (VPL * sizeof(T) <= 32)
       if (depth >= 16) return launch_deep<T, VPL, 16>(a, lpr);

@@ -94,9 +94,10 @@ def test_tune_fp32_twin(be64):
     m.delete()
 
 
+@pytest.mark.parametrize("lane_bytes", [0, 32])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("nvec", [1, 4, 8, 16])
-def test_spmv_depth_bitexact(be64, be32, dtype, nvec):
+def test_spmv_depth_bitexact(be64, be32, dtype, nvec, lane_bytes):
     """depths 16 and 32 (the next batch's indices prefetched; a whole 32-entry
     row's gathers in flight) and depth 8 give the same product bit for bit, on rows of 32 entries, of 40 entries (the
     second batch of a row mostly empty) and of at most 7 (a 3D Laplacian:
@@ -112,6 +113,8 @@ def test_spmv_depth_bitexact(be64, be32, dtype, nvec):
         x = be.mvec(m, nvec, X)
         out = {}
         try:
+            be.L.phist_amd_set_spmv_lane_bytes(lane_bytes, ctypes.byref(fl))
+            assert fl.value == 0
             for depth in (8, 16, 32):
                 be.L.phist_amd_set_spmv_depth(depth, ctypes.byref(fl))
                 assert fl.value == 0
@@ -121,6 +124,7 @@ def test_spmv_depth_bitexact(be64, be32, dtype, nvec):
                 y.delete()
         finally:
             be.L.phist_amd_set_spmv_depth(0, ctypes.byref(fl))
+            be.L.phist_amd_set_spmv_lane_bytes(0, ctypes.byref(fl))
         assert np.array_equal(out[8], out[16]) and np.array_equal(out[8], out[32])
         want = ol.spmv(ref, 1.0, X.astype(float), 0.0, np.zeros((n, nvec)))
         num = np.linalg.norm(out[32].astype(float) - want, axis=0)
