@@ -101,6 +101,11 @@ struct Runtime {
   // entries per row whose val/col loads and x gathers the SpMV issues
   // together (8 or 32; < 0: the matrix's tuned value, else 8)
   int spmv_depth = -1;
+  // y of the SpMV written with non-temporal stores, which keeps L2 for the
+  // gathered x rows: 1 always, 0 never, -1 (default) when y spans at least
+  // kNtYBytes (more than the L2s hold, so its next reader cannot find it there)
+  int spmv_nt_y = -1;
+  static constexpr int64_t kNtYBytes = int64_t(64) << 20;
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
   // ... when the wider side has at least gram_mfma_wide columns, or an

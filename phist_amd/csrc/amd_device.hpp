@@ -60,6 +60,26 @@ __device__ inline void vstore(T* p, const T (&in)[N]) {
   }
 }
 
+// vstore with non-temporal stores (the written lines are not kept in the
+// caches): 16-byte vectors where N scalars make whole vectors, else scalars
+template <typename T, int N>
+__device__ inline void vstore_nt(T* p, const T (&in)[N]) {
+  constexpr int V = 16 / sizeof(T);
+  if constexpr (N % V == 0) {
+    typedef T vec_t __attribute__((ext_vector_type(V)));
+#pragma unroll
+    for (int c = 0; c < N / V; c++) {
+      vec_t v;
+#pragma unroll
+      for (int e = 0; e < V; e++) v[e] = in[c * V + e];
+      __builtin_nontemporal_store(v, reinterpret_cast<vec_t*>(p) + c);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < N; e++) __builtin_nontemporal_store(in[e], p + e);
+  }
+}
+
 // sum over the lanes of a wave64 that share (lane % STRIDE)
 template <int STRIDE, typename T>
 __device__ inline T wave_sum_strided(T v) {

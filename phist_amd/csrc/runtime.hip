@@ -86,6 +86,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_LANE_BYTES")) r.spmv_lane_bytes = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_DEPTH")) r.spmv_depth = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_SPMV_NT_Y")) r.spmv_nt_y = atoi(s) < 0 ? -1 : (atoi(s) != 0 ? 1 : 0);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA_MIN_M")) r.mix_mfma_min_m = std::max(1, atoi(s));

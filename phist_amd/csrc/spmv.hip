@@ -63,7 +63,10 @@ inline int spmv_table_blocks_per_cu(double nnz_per_row, int nvec) {
   return 6;
 }
 
-enum : int { F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16, F_GWW = 32, F_GVW = 64, F_NOSTORE = 128 };
+enum : int {
+  F_SHIFT = 1, F_BETA = 2, F_DWW = 4, F_DVW = 8, F_AXPY = 16, F_GWW = 32, F_GVW = 64, F_NOSTORE = 128,
+  F_NTY = 256  // y written with non-temporal stores
+};
 
 // v_mfma_{f64,f32}_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // and holds four results of the 16x16 product in column l%16; the row of
@@ -321,7 +324,12 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 #pragma unroll
         for (int e = 0; e < VPL; e++) yn[e] = a.alpha * acc[e];
       }
-      if (!(flags & F_NOSTORE)) vstore<T, VPL>(yp, yn);
+      if (!(flags & F_NOSTORE)) {
+        if (flags & F_NTY)
+          vstore_nt<T, VPL>(yp, yn);
+        else
+          vstore<T, VPL>(yp, yn);
+      }
       if (flags & F_DWW) {
 #pragma unroll
         for (int e = 0; e < VPL; e++) dww[e] = fma(yn[e], yn[e], dww[e]);
@@ -591,6 +599,9 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   a.flags = (req.has_shift ? F_SHIFT : 0) | (req.has_beta ? F_BETA : 0) | (req.dot_ww ? F_DWW : 0) |
             (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0) | (req.gram_ww_dev ? F_GWW : 0) |
             (req.gram_vw_dev ? F_GVW : 0) | ((req.no_store || !y) ? F_NOSTORE : 0);
+  if (y && (rt().spmv_nt_y == 1 ||
+            (rt().spmv_nt_y < 0 && (int64_t)sizeof(T) * y->nrows * nvec >= Runtime::kNtYBytes)))
+    a.flags |= F_NTY;
   if (req.blocks_per_cu >= 0)
     a.blocks_per_cu = req.blocks_per_cu;
   else if (rt().spmv_blocks_per_cu >= 0)
