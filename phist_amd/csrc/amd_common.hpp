@@ -103,7 +103,9 @@ struct Runtime {
   int spmv_depth = -1;
   // y of the SpMV written with non-temporal stores, which keeps L2 for the
   // gathered x rows: 1 always, 0 never, -1 (default) when y spans at least
-  // kNtYBytes (more than the L2s hold, so its next reader cannot find it there)
+  // kNtYBytes (more than the L2s hold, so its next reader cannot find it
+  // there) and its rows are shorter than a 128-byte line (measured: rows of
+  // 32 and 64 bytes gain 1 %, rows of 128 and 256 bytes lose 4-5 %)
   int spmv_nt_y = -1;
   static constexpr int64_t kNtYBytes = int64_t(64) << 20;
   // Gram products with both operands >= 16 columns run on the matrix cores

@@ -599,8 +599,8 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   a.flags = (req.has_shift ? F_SHIFT : 0) | (req.has_beta ? F_BETA : 0) | (req.dot_ww ? F_DWW : 0) |
             (req.dot_vw ? F_DVW : 0) | (req.axpy_u ? F_AXPY : 0) | (req.gram_ww_dev ? F_GWW : 0) |
             (req.gram_vw_dev ? F_GVW : 0) | ((req.no_store || !y) ? F_NOSTORE : 0);
-  if (y && (rt().spmv_nt_y == 1 ||
-            (rt().spmv_nt_y < 0 && (int64_t)sizeof(T) * y->nrows * nvec >= Runtime::kNtYBytes)))
+  if (y && (rt().spmv_nt_y == 1 || (rt().spmv_nt_y < 0 && (int64_t)sizeof(T) * y->nrows * nvec >= Runtime::kNtYBytes &&
+                                     (int)sizeof(T) * nvec < 128)))
     a.flags |= F_NTY;
   if (req.blocks_per_cu >= 0)
     a.blocks_per_cu = req.blocks_per_cu;
