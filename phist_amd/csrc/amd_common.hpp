@@ -154,6 +154,7 @@ struct Runtime {
   void* fixed_work = nullptr;
   size_t fixed_work_bytes = 0;
   bool gram_rows = true;  // narrow unaligned Grams: several columns per lane (k_gram_rows)
+  bool gram_sets = true;  // self Grams of 2, 4 or 8 columns: row sets on the matrix cores (k_gram_mfma_sets)
   // optional timing of the main kernel of each hot-path call (HIP events on
   // the library stream around that launch only)
   bool ktimer_on = false;

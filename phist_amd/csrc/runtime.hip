@@ -96,6 +96,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_GMRES_CGS")) r.gmres_cgs = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_SERVE_MAX_REPS")) r.serve_max_reps = std::max(1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_GRAM_ROWS")) r.gram_rows = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_GRAM_SETS")) r.gram_sets = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH")) r.minres_graphs = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_ITERS_PER_CALL")) r.minres_iters_per_call = std::max(0, atoi(s));
   if (const char* s = getenv("PHIST_AMD_MINRES_GRAPH_LEN")) r.minres_graph_len = std::max(1, atoi(s));

@@ -260,6 +260,68 @@ __global__ void __launch_bounds__(kBlock) k_gram_mfma(const GramArgs<T> g) {
   });
 }
 
+// Self Gram V'V of a narrow block (KC = 2, 4 or 8 columns, row stride even,
+// rows 2-scalar aligned) on the matrix cores with every lane loading: lane
+// cc of an MFMA column group loads the column pair p = cc % P (P = KC/2) of
+// a row of row set s = cc / P, so one 16x16x4 step covers 4 rows of each of
+// RS = 16/P row sets (64/P rows, all 64 lanes loading 2 scalars) instead of
+// 4 rows on KC of 16 lanes.  Tile t of the pair (.x / .y) holds the even /
+// odd columns; of each 16x16 product only the blocks of equal row sets are
+// kept (entry (s*P+p, s*P+p') = sum over the set's rows of x(2p+ta) x(2p'+tb)).
+// At the end a wave adds the RS blocks of each output in set order through
+// LDS, so every output is a fixed-order sum.
+template <typename T, int KC>
+__global__ void __launch_bounds__(kBlock) k_gram_mfma_sets(const GramArgs<T> g) {
+  using M = Mfma<T>;
+  constexpr int P = KC / 2, RS = 16 / P, ROWS = 64 / P;  // rows per wave and step
+  constexpr int UNR = 4;                                   // steps in flight per wave
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kk = lane >> 4, cc = lane & 15;
+  const int sset = cc / P, pr = cc % P;
+  typename M::acc_t acc[3];  // tile pairs (0,0), (0,1), (1,1)
+#pragma unroll
+  for (int t = 0; t < 3; t++) acc[t] = typename M::acc_t{0, 0, 0, 0};
+  const TileWalk tw = xcd_tiles(g.ntiles);
+  for (int64_t t = tw.t; t < tw.end; t += UNR * tw.stride) {
+    T2v<T> v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t tu = t + u * tw.stride;
+      const int64_t r = tu * (4 * ROWS) + wv * ROWS + sset * 4 + kk;
+      v[u] = T2v<T>{T(0), T(0)};
+      if (tu < tw.end && r < g.nrows) v[u] = *reinterpret_cast<const T2v<T>*>(g.a + r * g.lda + 2 * pr);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      acc[0] = M::op(v[u].x, v[u].x, acc[0]);
+      acc[1] = M::op(v[u].x, v[u].y, acc[1]);
+      acc[2] = M::op(v[u].y, v[u].y, acc[2]);
+    }
+  }
+  // the wave's three 16x16 tiles through LDS, then output (i, j) = sum over
+  // the row sets in order, owned by lane i*KC + j
+  extern __shared__ char gram_lds[];
+  T* tiles = reinterpret_cast<T*>(gram_lds) + (size_t)wv * 3 * 256;
+  T* red = reinterpret_cast<T*>(gram_lds) + (size_t)(kBlock / 64) * 3 * 256;
+#pragma unroll
+  for (int t = 0; t < 3; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) tiles[t * 256 + M::row(lane, r) * 16 + cc] = acc[t][r];
+  __syncthreads();
+  T val = T(0);
+  const int i = lane / KC, j = lane % KC;
+  if (lane < KC * KC) {
+    const int ta = i & 1, pa = i >> 1, tb = j & 1, pb = j >> 1;
+    // tile (ta, tb) for ta <= tb; (1, 0) is the transpose of (0, 1)
+    const int tt = ta + tb, m0 = ta <= tb ? pa : pb, n0 = ta <= tb ? pb : pa;
+#pragma unroll
+    for (int s2 = 0; s2 < RS; s2++) val += tiles[tt * 256 + (s2 * P + m0) * 16 + (s2 * P + n0)];
+  }
+  block_tile_sum(red, KC * KC, g.partials + (int64_t)blockIdx.x * KC * KC, [&](auto add) {
+    if (lane < KC * KC) add(j * KC + i, val);
+  });
+}
+
 // C(r,c) = alpha * S + beta * C with S summed over the block partials in a
 // fixed order; S is stored c*na + r, transposed when the operands were
 // swapped.  One wave per output (four per workgroup, finalize_blocks(nout)
@@ -357,6 +419,41 @@ static int gram_mfma_launch(GramArgs<T>& g) {
   return grid;
 }
 
+template <typename T, int KC>
+static int gram_sets_launch(GramArgs<T>& g) {
+  auto kern = k_gram_mfma_sets<T, KC>;
+  constexpr int ROWS = 64 / (KC / 2);
+  g.ntiles = (g.nrows + 4 * ROWS - 1) / (4 * ROWS);
+  const size_t lds = sizeof(T) * ((size_t)(kBlock / 64) * 3 * 256 + KC * KC);
+  const int grid = persistent_blocks(kern, lds, g.ntiles, rt().gram_blocks_per_cu);
+  g.partials = (T*)device_work(sizeof(T) * (size_t)grid * KC * KC);
+  ktimer_begin();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, g);
+  PHX_HIP(hipGetLastError());
+  ktimer_end();
+  return grid;
+}
+
+// self Gram of a narrow block (2, 4 or 8 columns, pairs 2-scalar aligned)
+// by row sets; false when the shape does not fit
+template <typename T>
+static bool gram_self_sets(T alpha, const T* a, int64_t lda, int k, int64_t nrows, T beta, T* Cdev, int ldc) {
+  if (!(k == 2 || k == 4 || k == 8) || lda % 2 != 0 || ((uintptr_t)a % (2 * sizeof(T))) != 0) return false;
+  GramArgs<T> g;
+  g.nrows = nrows;
+  g.a = a;
+  g.lda = lda;
+  g.na = k;
+  g.b = a;
+  g.ldb = lda;
+  g.nb = k;
+  const int nparts = k == 2 ? gram_sets_launch<T, 2>(g) : k == 4 ? gram_sets_launch<T, 4>(g) : gram_sets_launch<T, 8>(g);
+  hipLaunchKernelGGL(k_gram_finalize<T>, dim3(finalize_blocks(k * k)), dim3(kBlock), 0, rt().stream, g.partials,
+                     nparts, k, k, false, alpha, beta, Cdev, ldc);
+  PHX_HIP(hipGetLastError());
+  return true;
+}
+
 // S = a'b on the matrix cores for na, nb <= 32 (zero-padded to 16-column
 // tiles), written to C as C = alpha*S + beta*C; Cmirror, when given,
 // receives the transposed block from the same partial sums
@@ -449,6 +546,11 @@ void gemm_gram(T alpha, const Mvec<T>* V, const Mvec<T>* W, T beta, SdMat<T>* C,
   const bool self = V->root == W->root && V->jofs == W->jofs && m == k;
   const int wide = std::max(m, k);
   const bool strided = V->lda != m || W->lda != k;
+  if (rt().gram_mfma && self && rt().gram_sets &&
+      gram_self_sets<T>(alpha, V->dev(), V->lda, m, V->nrows, beta, C->dev(), ldc)) {
+    if (to_host) sd_from_device(C);
+    return;
+  }
   if (rt().gram_mfma && (wide >= rt().gram_mfma_wide || (strided && wide >= 4) || (self && wide >= 8))) {
     // matrix cores: 32 x 32 output blocks (the self Gram computes the upper
     // blocks and mirrors them)

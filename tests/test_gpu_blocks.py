@@ -140,9 +140,10 @@ def test_gram(be, m, k, beta):
         o.delete()
 
 
-@pytest.mark.parametrize("k", [16, 24, 32, 40, 64])
+@pytest.mark.parametrize("k", [2, 4, 8, 16, 24, 32, 40, 64])
 def test_gram_self_mfma(be, k):
-    """V'V on the matrix cores (upper tiles computed, mirrored) vs the oracle"""
+    """V'V on the matrix cores (upper tiles computed, mirrored; 2, 4 and 8 columns by row sets, every
+    lane loading a column pair) vs the oracle; 4099 rows leave a partial last step"""
     n = 4099
     V = np.random.default_rng(k).uniform(-1, 1, (n, k))
     mp = be.map(n)
@@ -177,6 +178,25 @@ def test_gram_narrow_views(be, j0, j1, k):
     bound = 8 * RTOL[be.dtype.type] * (np.abs(Vt).T @ np.abs(Wt))
     assert np.all(np.abs(C.get().astype(float) - want) <= bound)
     for o in (xv, x, w, C):
+        o.delete()
+
+
+@pytest.mark.parametrize("j0,k", [(2, 4), (4, 8), (6, 2), (3, 4)])
+def test_gram_self_narrow_views(be, j0, k):
+    """V'V of a column view of a 12-column block (row stride 12): aligned views take the row-set kernel,
+    the view at an odd offset the general path; both vs the oracle"""
+    n = 3333
+    X = np.random.default_rng(j0 + 10 * k).uniform(-1, 1, (n, 12))
+    mp = be.map(n)
+    x = be.mvec(mp, 12, X)
+    xv = x.view(j0, j0 + k - 1)
+    C = be.sdmat(k, k)
+    be.gram(1.0, xv, xv, 0.0, C)
+    Vt = X[:, j0:j0 + k].astype(be.dtype).astype(float)
+    want = ol.gram(1.0, Vt, Vt, 0.0, np.zeros((k, k)))
+    bound = 8 * RTOL[be.dtype.type] * (np.abs(Vt).T @ np.abs(Vt))
+    assert np.all(np.abs(C.get().astype(float) - want) <= bound)
+    for o in (xv, x, C):
         o.delete()
 
 
