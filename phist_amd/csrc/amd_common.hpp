@@ -197,6 +197,10 @@ void* pinned_alloc(size_t bytes);
 void pinned_release(void* p, size_t bytes);
 void pool_trim();
 // brackets one kernel launch when the kernel timer is on
+// resident workgroups per CU of a kernel launched with `threads` threads
+// and lds_bytes of dynamic LDS (the runtime's occupancy query, asked once
+// per kernel, workgroup size, LDS size and device; at least 1)
+int occupancy_per_cu(const void* kernel, int threads, size_t lds_bytes);
 void ktimer_begin();
 void ktimer_end();  // synchronises the event and accumulates the time
 

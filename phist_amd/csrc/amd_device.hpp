@@ -93,9 +93,7 @@ __device__ inline T wave_sum_strided(T v) {
 template <typename K>
 inline int persistent_blocks(K kernel, size_t lds_bytes, int64_t ntiles, int max_per_cu = 0) {
   Runtime& r = rt();
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds_bytes) != hipSuccess || per_cu < 1)
-    per_cu = 1;
+  int per_cu = occupancy_per_cu(reinterpret_cast<const void*>(kernel), kBlock, lds_bytes);
   if (max_per_cu > 0) per_cu = std::min(per_cu, max_per_cu);
   int64_t g = (int64_t)r.num_cu * per_cu;
   if (r.grid_blocks > 0) g = std::min<int64_t>(g, r.grid_blocks);

@@ -8,6 +8,7 @@
 #include "amd_common.hpp"
 #include <cstdarg>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 #include <cstdlib>
@@ -149,6 +150,27 @@ void* host_work(size_t bytes) {
 }
 
 void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
+
+// The occupancy query costs microseconds of host time per call, as much as a
+// launch; small solves launch thousands of kernels, so the answers are kept.
+int occupancy_per_cu(const void* kernel, int threads, size_t lds_bytes) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, int, size_t>, int> known;
+  const auto key = std::make_tuple(rt().device, kernel, threads, lds_bytes);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(key);
+    if (it != known.end()) return it->second;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds_bytes) != hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  known[key] = per_cu;
+  return per_cu;
+}
 
 bool stage_upload_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t row_bytes, size_t cols) {
   Runtime& r = rt();
