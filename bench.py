@@ -31,6 +31,11 @@ import time
 # is run (set before any OpenMP runtime starts; a caller's own settings win)
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# the JD CPU baseline alternates OpenMP kernels (the reference's SpMV) with
+# BLAS calls (numpy's OpenBLAS): threads of either runtime go to sleep at once
+# after their region instead of spinning on the cores the other one needs
+os.environ.setdefault("KMP_BLOCKTIME", "0")
+os.environ.setdefault("OPENBLAS_THREAD_TIMEOUT", "4")
 
 import numpy as np
 
@@ -291,14 +296,47 @@ def jada_bench(be, args):
               file=sys.stderr, flush=True)
     speed = results["fp64"]["seconds"] / results["mixed"]["seconds"] if len(results) == 2 else None
     main = results.get("mixed", results.get("fp64"))
+    cpu = None
+    if not args.no_cpu and (prob.startswith("lap") or args.cpu_jada):
+        cpu = jada_cpu_baseline(A, cfg, args.inner)
     out = {"metric": "JD time-to-1e-8 mixed vs FP64", "value": main["seconds"], "unit": "s",
            "n_gpus": 1, "higher_is_better": False, "dtype": "f64 outer / f32 inner",
            "data": "synthetic (generated matrix, KISS random start vectors)",
            "config": {"workload": name, "which": "SR", "tol": 1e-8, **cfg,
                       **({} if tune is None else {"autotuned": tune})},
-           **results, "speedup_mixed_vs_fp64": None if speed is None else round(speed, 3)}
+           **results, "speedup_mixed_vs_fp64": None if speed is None else round(speed, 3),
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     A.delete()
+
+
+def jada_cpu_baseline(A, cfg, inner):
+    """The same JD solve on the host cores: oracle/jada_port.py (a numpy port of
+    phist_subspacejada with blocked-GMRES correction solves) over the reference's
+    own OpenMP SpMV (flang-built builtin dspmvm, else the oracle's C SpMV), from
+    the first start vector of the GPU runs (the KISS stream from its initial
+    state), FP64 throughout; one solve, wall clock."""
+    import oracle_lib as ol
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import jada_port
+    if inner != "GMRES":
+        return None  # the port solves the correction equations by GMRES only
+    ro, ci, va = A.crs()
+    n = len(ro) - 1
+    op = jada_port.CrsOperator(n, ro, ci, va)
+    v0 = ol.mvec_random_sequence([(n, 1)])[0][:, 0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print(f"bench: timing the CPU JD ({op.kind} SpMV, {threads} threads)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ev, res, nconv, niter, _q = jada_port.subspacejada(
+        op, v0, cfg["num_eigs"], cfg["block_size"], cfg["min_bas"], cfg["max_bas"], 1e-8, cfg["inner_max_iters"],
+        max_iters=500)
+    secs = time.perf_counter() - t0
+    return {"value": round(secs, 3), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"the whole solve from start vector 0 (FP64): {niter} outer iterations, {nconv} converged, "
+                      f"max residual {float(np.max(res)):.2e}, lowest {float(np.min(ev[:max(1, nconv)])):.10f}; "
+                      f"SpMV: {'the reference builtin dspmvm (flang, OpenMP)' if op.kind == 'reference' else 'oracle C'}",
+            **cpu_info_fields(threads)}
 
 
 def tune_bench(be, args):
@@ -735,6 +773,8 @@ def main():
     ap.add_argument("--rate", type=float, default=40.0, help="--op serve: mean request arrivals per second")
     ap.add_argument("--duration", type=float, default=20.0, help="--op serve: seconds of arrivals")
     ap.add_argument("--starts", type=int, default=5, help="--op jada: start vectors averaged per precision")
+    ap.add_argument("--cpu-jada", action="store_true",
+                    help="--op jada: time the CPU port also for the spin chain (default: Laplacians only)")
     ap.add_argument("--max-iter", type=int, default=None, help="--op minres: iteration limit (default 3000)")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="--op jada: run the FP64 solve, the mixed-precision one, or both")
