@@ -452,6 +452,9 @@ def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
         print(f"bench: minres {label} (full sheet): {dt:.2f} s, {iters} iterations, true relres max "
               f"{tr.max():.3e} median {np.median(tr):.3e}", file=sys.stderr, flush=True)
         X.delete()
+    cpu = {}
+    if not args.no_cpu:
+        cpu["full_sheet"] = minres_cpu_sample(A, B.get(), shifts, tol)
     B.delete()
     A.delete()
     # ---- a sheet where the tolerance is reachable: time to 1e-6 with slot refill ----
@@ -479,6 +482,8 @@ def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
               f"{int(np.median(it))}, true relres max {tr.max():.2e} ({int(np.sum(tr <= tol))}/{nsys} <= tol)",
               file=sys.stderr, flush=True)
         X.delete()
+    if not args.no_cpu:
+        cpu["small_sheet"] = minres_cpu_solve(A, B.get(), shifts, tol, 60000)
     B.delete()
     A.delete()
     out = {"metric": "batched shifted MINRES (32 shifts x 32 random unit RHS, tol 1e-6)",
@@ -488,8 +493,50 @@ def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
            "config": {"workload": f"honeycomb {W}x{L} ({N} rows), nearest neighbours, on-site U(-0.5,0.5), "
                                   f"{nsys} shifts in [-2.5,2.5], random unit RHS, tol {tol}; full sheet run for "
                                   f"{max_iter} iterations; time to tol on {Ws}x{Ls} with {slots} slots"},
-           "autotuned": tune, "full_sheet": results["full"], "small_sheet": results["small"]}
+           "autotuned": tune, "full_sheet": results["full"], "small_sheet": results["small"],
+           "cpu_baseline": cpu or None}
     print(json.dumps(out), flush=True)
+
+
+def _minres_port(A):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import jada_port
+    import minres_port
+    ro, ci, va = A.crs()
+    op = jada_port.CrsOperator(len(ro) - 1, ro, ci, va)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return minres_port, op, threads
+
+
+def minres_cpu_sample(A, Bh, shifts, tol, iters=4):
+    """FP64 MINRES iterations of the same block on the host cores (oracle/minres_port.py over the
+    reference's flang-built OpenMP SpMV, vector passes on row blocks in a thread pool): the bounded
+    sample is `iters` iterations, the first untimed; reported per iteration, the unit of the GPU line"""
+    minres_port, op, threads = _minres_port(A)
+    print(f"bench: timing {iters} CPU MINRES iterations ({op.kind} SpMV, {threads} threads)", file=sys.stderr,
+          flush=True)
+    marks = []
+    minres_port.blocked_minres(op, shifts, np.ascontiguousarray(Bh), tol, iters, threads=threads,
+                               on_iteration=lambda k: marks.append(time.perf_counter()))
+    per = (marks[-1] - marks[0]) / max(1, len(marks) - 1)
+    return {"value": round(1e3 * per, 2), "unit": "ms/iteration (FP64)", "cores": threads, "kind": "port",
+            "sample": f"iterations 2..{iters} of the same {Bh.shape[1]} systems on all {Bh.shape[0]} rows (FP64); "
+                      f"SpMV: {'the reference builtin dspmvm (flang, OpenMP)' if op.kind == 'reference' else 'oracle C'}",
+            **cpu_info_fields(threads)}
+
+
+def minres_cpu_solve(A, Bh, shifts, tol, max_iter):
+    """time to tol of the same systems on the host cores, all in one block (FP64)"""
+    minres_port, op, threads = _minres_port(A)
+    pool = max(1, min(threads, Bh.shape[0] // 65536))  # small sheets: no row blocks
+    t0 = time.perf_counter()
+    _x, its, rr = minres_port.blocked_minres(op, shifts, np.ascontiguousarray(Bh), tol, max_iter, threads=pool)
+    secs = time.perf_counter() - t0
+    return {"value": round(secs, 3), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"the whole solve, {Bh.shape[1]} systems in one block (FP64): iterations max {int(its.max())} "
+                      f"median {int(np.median(its))}, {int(np.sum(rr <= tol))}/{Bh.shape[1]} at tol by the recurrence; "
+                      f"vector passes on {pool} thread(s)",
+            **cpu_info_fields(threads)}
 
 
 def gmres_bench(be, args, n=128, nsys=16, tol=1e-8, restart=30, max_iter=600):
@@ -524,15 +571,44 @@ def gmres_bench(be, args, n=128, nsys=16, tol=1e-8, restart=30, max_iter=600):
         print(f"bench: gmres {label}: {dt:.2f} s, max {iters} Arnoldi steps, {unconv} unconverged",
               file=sys.stderr, flush=True)
         X.delete()
+    cpu = None
+    if not args.no_cpu:
+        cpu = gmres_cpu_baseline(A, B.get(), shifts, restart)
     out = {"metric": "blocked shifted GMRES time (16 shifts, tol 1e-8)", "value": results["fp64"]["seconds"],
            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64 and f32",
            "data": "synthetic (3D Laplacian, KISS random right-hand sides)",
            "config": {"workload": f"3D Laplacian {n}^3 ({N} rows), {nsys} shifts in [0.05,1.0], GMRES({restart}), "
                                   f"tol {tol}, maxIter {max_iter}, zero initial guess"},
-           "fp64": results["fp64"], "fp32": results["fp32"]}
+           "fp64": results["fp64"], "fp32": results["fp32"], "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     B.delete()
     A.delete()
+
+
+def gmres_cpu_baseline(A, Bh, shifts, restart):
+    """The first GMRES(restart) cycle of the same systems on the host cores (FP64): the blocked
+    GMRES of oracle/jada_port.py (one basis per system, classical Gram-Schmidt twice) over the
+    reference's flang-built OpenMP SpMV and OpenBLAS; reported per Arnoldi step, the unit of the
+    GPU runs' ms_per_step (the bounded sample: one cycle of `restart` steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import jada_port
+    ro, ci, va = A.crs()
+    op = jada_port.CrsOperator(len(ro) - 1, ro, ci, va)
+    del ro, ci, va
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print(f"bench: timing one CPU GMRES({restart}) cycle ({op.kind} SpMV, {threads} threads)", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    _x, its, rr = jada_port.blocked_gmres(op, list(shifts), np.ascontiguousarray(Bh), None,
+                                          np.full(Bh.shape[1], 1e-8), restart, restart)
+    secs = time.perf_counter() - t0
+    steps = int(its.max())
+    return {"value": round(1e3 * secs / max(1, steps), 2), "unit": "ms per Arnoldi step", "cores": threads,
+            "kind": "port", "seconds": round(secs, 3),
+            "sample": f"the first GMRES({restart}) cycle of all {Bh.shape[1]} systems ({steps} Arnoldi steps, FP64, "
+                      f"relative residual after it {float(rr.min()):.1e}..{float(rr.max()):.1e}); SpMV: "
+                      f"{'the reference builtin dspmvm (flang, OpenMP)' if op.kind == 'reference' else 'oracle C'}",
+            **cpu_info_fields(threads)}
 
 
 def carp_bytes(nnz, n, nvec, s=8):
