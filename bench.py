@@ -215,8 +215,65 @@ def tall_skinny(be, args):
            "config": {"workload": f"{args.op} on 16,777,216 x {k} U(-1,1)", "bytes_per_op": nbytes,
                       "kernel_ms_per_op": round(k_ms, 4), "kernel_launches_timed": kcount.value},
            "roofline": {"bound": "hbm", "achieved": round(nbytes / (k_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(nbytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+                        "unit": "GB/s", "frac": round(nbytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "cpu_baseline": None if args.no_cpu else tall_skinny_cpu_baseline(args.op, v.get(), nbytes)}
     print(json.dumps(out), flush=True)
+
+
+def tall_skinny_cpu_baseline(op, Vh, nbytes, reps=5):
+    """the same operation on the host cores, same byte model, median of `reps` after one untimed run:
+    gram: V'V by OpenBLAS (the reference's gemm_kernels_sC.F90 needs its CMake-generated header);
+    cholqr: that Gram, the k x k Cholesky (LAPACK), then V <- V R^-1 by the reference's flang-built OpenMP
+    dgemm_sB_generic_inplace; mvsd: W = V C and W'W by the reference's dgemm_sB_augmented_generic"""
+    import oracle_lib as ol
+    ref = ol.ref_leaf()
+    n, k = Vh.shape
+    I, D = ctypes.c_int, ctypes.c_double
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import blocks_port
+    if op == "gram":
+        def run():
+            return blocks_port.gram(Vh, threads)
+        how = (f"V'V as row blocks on {threads} threads, each by single-threaded OpenBLAS, added in block order "
+               "(oracle/blocks_port.py; the reference's gemm_kernels_sC.F90 is unbuildable here)")
+    elif op == "mvsd":
+        if ref is None:
+            return None
+        Cm = np.asfortranarray(np.random.default_rng(3).uniform(-1, 1, (k, k)))
+        W = np.zeros((n, k))
+        Dm = np.zeros((k, k), order="F")
+
+        def run():
+            ref.dgemm_sb_augmented_generic_(ctypes.byref(I(n)), ctypes.byref(I(k)), ctypes.byref(I(k)),
+                                            ctypes.byref(D(1.0)), ol.p(Vh), ctypes.byref(I(k)), ol.p(Cm),
+                                            ctypes.byref(D(0.0)), ol.p(W), ctypes.byref(I(k)), ol.p(Dm))
+        how = "the reference's dgemm_sB_augmented_generic (flang, OpenMP)"
+    else:
+        if ref is None:
+            return None
+        V0 = Vh.copy()
+
+        def run():
+            np.copyto(Vh, V0)  # fresh input, as on the GPU (outside the byte model there too)
+            t = time.perf_counter()
+            R = np.linalg.cholesky(blocks_port.gram(Vh, threads)).T
+            Rinv = np.asfortranarray(np.linalg.inv(R))
+            ref.dgemm_sb_generic_inplace_(ctypes.byref(I(n)), ctypes.byref(I(k)), ctypes.byref(I(k)), ol.p(Vh),
+                                          ctypes.byref(I(k)), ol.p(Rinv))
+            return time.perf_counter() - t
+        how = "Gram as for the gram op (oracle/blocks_port.py), k x k Cholesky by LAPACK, V R^-1 in place by " \
+              "the reference's dgemm_sB_generic_inplace (flang, OpenMP)"
+    run()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = run()
+        times.append(r if isinstance(r, float) else time.perf_counter() - t0)
+    sec = float(np.median(times))
+    return {"value": round(nbytes / sec / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+            "ms_per_op": round(1e3 * sec, 3),
+            "sample": f"the whole {n} x {k} block (FP64), median of {reps}: {how}", **cpu_info_fields(threads)}
 
 
 def jada_bench(be, args):
