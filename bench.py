@@ -300,13 +300,13 @@ def jada_bench(be, args):
         # matrix and on its FP32 copy (the correction solves of the mixed run)
         pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         nv, dms, tms, st = (np.array([cfg["block_size"]], np.int32), np.zeros(1), np.zeros(1),
-                            np.zeros(5, np.int32))
+                            np.zeros(6, np.int32))
         iflag = ctypes.c_int(0)
         t0 = time.perf_counter()
         be.L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 4, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
         if iflag.value:
             raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
-        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]),
+        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "col_block": int(st[5]),
                 "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
                 "tune_s": round(time.perf_counter() - t0, 2)}
         print(f"bench: autotuner (FP64 matrix; its FP32 copy tuned too) {tune}", file=sys.stderr, flush=True)
@@ -417,7 +417,7 @@ def tune_bench(be, args):
         A = make()
         t_build = time.perf_counter() - t0
         n = len(widths)
-        nv, dms, tms, st = np.array(widths, np.int32), np.zeros(n), np.zeros(n), np.zeros(5 * n, np.int32)
+        nv, dms, tms, st = np.array(widths, np.int32), np.zeros(n), np.zeros(n), np.zeros(6 * n, np.int32)
         t0 = time.perf_counter()
         L.phist_Damd_sparseMat_tune(A.h, n, pp(nv), 1 | 2, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
         if iflag.value:
@@ -425,11 +425,11 @@ def tune_bench(be, args):
         t_tune = time.perf_counter() - t0
         rows = []
         for i, w in enumerate(widths):
-            C, sg, lb, bpc, depth = (int(v) for v in st[5 * i:5 * i + 5])
+            C, sg, lb, bpc, depth, cblk = (int(v) for v in st[6 * i:6 * i + 6])
             gb = spmv_bytes(A.nnz, A.nrows, w) / 1e9
             rows.append({"nvec": w, "untuned_ms": round(float(dms[i]), 4), "tuned_ms": round(float(tms[i]), 4),
                          "gain": round(float(dms[i] / tms[i]), 3), "C": C, "sigma": sg, "lane_bytes": lb,
-                         "blocks_per_cu": bpc, "depth": depth, "tuned_GBs": round(gb / (tms[i] * 1e-3), 1)})
+                         "blocks_per_cu": bpc, "depth": depth, "col_block": cblk, "tuned_GBs": round(gb / (tms[i] * 1e-3), 1)})
         out = {"metric": "SpMV autotuner: kernel ms per product, untuned vs tuned", "workload": name,
                "rows": A.nrows, "nnz": A.nnz, "build_s": round(t_build, 2), "tune_s": round(t_tune, 2),
                "widths": rows}
@@ -477,14 +477,14 @@ def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
     # the library's autotuner picks the SELL layout and launch settings of the
     # 32-column product for the matrix and its FP32 copy (set-up, not timed)
     pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    nv, dms, tms, st = np.array([nsys], np.int32), np.zeros(1), np.zeros(1), np.zeros(5, np.int32)
+    nv, dms, tms, st = np.array([nsys], np.int32), np.zeros(1), np.zeros(1), np.zeros(6, np.int32)
     iflag = ctypes.c_int(0)
     t0 = time.perf_counter()
     be.L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 1 | 4, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
     if iflag.value:
         raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
     tune = {"C": int(st[0]), "sigma": int(st[1]), "lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]),
-            "depth": int(st[4]), "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
+            "depth": int(st[4]), "col_block": int(st[5]), "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
             "tune_s": round(time.perf_counter() - t0, 2)}
     print(f"bench: autotuner {tune}", file=sys.stderr, flush=True)
     m = A.domain_map
@@ -930,6 +930,8 @@ def main():
                     help="--op spmv: SpMV workgroups per CU (-1 = library table)")
     ap.add_argument("--depth", type=int, default=0, choices=[0, 8, 16, 32],
                     help="--op spmv: entries per row whose loads the SpMV issues together (0: tuned / default)")
+    ap.add_argument("--col-block", type=int, default=-1,
+                    help="--op spmv: block columns per launch (0 = one launch, -1 = as tuned)")
     ap.add_argument("--no-tune", action="store_true",
                     help="--op spmv / jada: skip the autotuner (launch settings from the library's table)")
     args = ap.parse_args()
@@ -979,6 +981,8 @@ def main():
     L.phist_amd_set_spmv_blocks_per_cu(args.blocks_per_cu, ctypes.byref(iflag))
     if not iflag.value:
         L.phist_amd_set_spmv_depth(args.depth, ctypes.byref(iflag))
+    if not iflag.value:
+        L.phist_amd_set_spmv_col_block(args.col_block, ctypes.byref(iflag))
     if iflag.value:
         raise RuntimeError(f"SpMV launch settings rejected (iflag={iflag.value})")
     N, nvec = args.rows, args.nvec
@@ -992,16 +996,17 @@ def main():
     nbytes = spmv_bytes(A.nnz, N, nvec)
     print(f"bench: matrix {N} rows, {A.nnz} nnz built in {t_build:.1f} s; nvec={nvec}", file=sys.stderr, flush=True)
     tune = None
-    if not args.no_tune and args.lane_bytes == 0 and args.blocks_per_cu < 0 and args.depth == 0:
+    if not args.no_tune and args.lane_bytes == 0 and args.blocks_per_cu < 0 and args.depth == 0 and args.col_block < 0:
         # the library's autotuner picks the launch settings for this width; the
         # layout stays the configuration's SELL-32-256
-        nv, dms, tms, st = np.array([nvec], np.int32), np.zeros(1), np.zeros(1), np.zeros(5, np.int32)
+        nv, dms, tms, st = np.array([nvec], np.int32), np.zeros(1), np.zeros(1), np.zeros(6, np.int32)
         pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         t0 = time.perf_counter()
         L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 0, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
         if iflag.value:
             raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
-        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "untuned_kernel_ms": round(float(dms[0]), 4),
+        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "col_block": int(st[5]),
+                "untuned_kernel_ms": round(float(dms[0]), 4),
                 "tuned_kernel_ms": round(float(tms[0]), 4), "tune_s": round(time.perf_counter() - t0, 2)}
         print(f"bench: autotuner {tune}", file=sys.stderr, flush=True)
 
@@ -1031,11 +1036,13 @@ def main():
     L.phist_amd_kernel_timer_read(ctypes.byref(kms), ctypes.byref(kcount), ctypes.byref(iflag))
     L.phist_amd_kernel_timer(0, ctypes.byref(iflag))
     ms_step = (t1 - t0) * 1e3 / args.steps
-    k_avg_ms = kms.value / max(1, kcount.value)
+    # kernel time per product: one launch, or one per column block when the
+    # tuner split the block columns (then kernel_launches_timed is a multiple of steps)
+    k_avg_ms = kms.value / max(1, args.steps)
     value = nbytes / (ms_step * 1e-3) / 1e9
     achieved = nbytes / (k_avg_ms * 1e-3) / 1e9
     default_run = args.halfband == 50000 and N == 4194304 and args.lane_bytes == 0 and args.blocks_per_cu < 0 \
-        and args.depth == 0 and not args.no_tune
+        and args.depth == 0 and args.col_block < 0 and not args.no_tune
     traffic = read_traffic(os.path.join(ROOT, "profiles", f"sell_spmv_fused_nvec{nvec}_traffic.txt")) \
         if default_run else None
 
@@ -1067,6 +1074,7 @@ def main():
             **({} if args.lane_bytes == 0 else {"lane_bytes": args.lane_bytes}),
             **({} if args.blocks_per_cu < 0 else {"blocks_per_cu": args.blocks_per_cu}),
             **({} if args.depth == 0 else {"depth": args.depth}),
+            **({} if args.col_block < 0 else {"col_block": args.col_block}),
             **({} if tune is None else {"autotuned": tune}),
         },
         "roofline": {

@@ -43,16 +43,25 @@ void phist_amd_set_spmv_lane_bytes(int bytes, int* iflag);
  * factors, and fall back to 8 otherwise.  Every choice
  * gives the same result bit for bit. */
 void phist_amd_set_spmv_depth(int slots, int* iflag);
+/* block columns per SpMV launch: ncols > 0 = at most that many (a multiple of
+ * the scalars each lane holds, else ignored), 0 = as many as 64 lane groups
+ * hold, -1 = the default: the matrix's tuned value, else 0.  Each further
+ * launch reads the matrix again; wide rows of x (256-byte rows at 32 FP64
+ * columns) gather half as many bytes per launch in two blocks of 16.  Every
+ * choice gives the same result bit for bit (column dots to rounding). */
+void phist_amd_set_spmv_col_block(int ncols, int* iflag);
 /* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16
  * widths) it times the product y = A*x on the device over the gather width
  * per lane (one scalar, or one, two or four 16-byte vectors: how many lanes
- * share a row), the workgroups per CU (1, 2, 3, 4, 6, 8) and, for rows of at
- * least 16 entries, the depth (8, 16 or 32, phist_amd_set_spmv_depth), and with
+ * share a row), the workgroups per CU (1, 2, 3, 4, 6, 8), for rows of at
+ * least 16 entries the depth (8, 16 or 32, phist_amd_set_spmv_depth), for more
+ * than 16 columns launches of 16 (and of 32 above 32) columns
+ * (phist_amd_set_spmv_col_block), and with
  * flags & 1 also over the SELL-C-sigma layout (C = 16, 32, 64; sigma = 1,
  * 256; the device copy is rebuilt from the host CRS for each).  The best
  * layout (least total time over the widths) is kept, and the best launch
  * settings of each width are stored on the matrix: later SpMVs of that width
- * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu / _depth set global
+ * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu / _depth / _col_block set global
  * values.  Choices are cached for the process by matrix signature (scalar
  * type, rows, entries, longest row, mean distance of entries from the
  * diagonal, block width); flags & 2 measures again instead of using the
@@ -60,7 +69,7 @@ void phist_amd_set_spmv_depth(int slots, int* iflag);
  * mixed-precision solvers (launch settings, same widths).  Outputs may be
  * NULL: default_ms[i] / tuned_ms[i] = fastest of three products with the
  * library's untuned choice on the incoming layout / with the chosen settings;
- * settings[5*i..5*i+4] = C, sigma, lane bytes, workgroups per CU, depth.  Products
+ * settings[6*i..6*i+5] = C, sigma, lane bytes, workgroups per CU, depth, column block.  Products
  * are identical bit for bit under every setting. */
 void phist_Damd_sparseMat_tune(phist_DsparseMat_ptr A, int n_nvec, const int* nvecs, int flags, double* default_ms,
                                double* tuned_ms, int* settings, int* iflag);

@@ -107,6 +107,9 @@ struct Runtime {
   // there) and its rows are shorter than a 128-byte line (measured: rows of
   // 32 and 64 bytes gain 1 %, rows of 128 and 256 bytes lose 4-5 %)
   int spmv_nt_y = -1;
+  // block columns per SpMV launch: > 0 at most that many, 0 as many as 64
+  // lane groups hold, -1 (default) the matrix's tuned value
+  int spmv_col_block = -1;
   static constexpr int64_t kNtYBytes = int64_t(64) << 20;
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
@@ -287,6 +290,7 @@ struct SpmvTuned {
   int lane_bytes = 0;
   int blocks_per_cu = 0;
   int depth = 8;  // entries per row whose loads are issued together (8 or 32)
+  int col_block = 0;  // block columns per launch (0: as many as 64 lane groups hold)
   double ms = 0.0;  // measured time per product when chosen
 };
 
@@ -395,6 +399,7 @@ struct SpmvRequest {
   int lane_bytes = -1;
   int blocks_per_cu = -1;
   int depth = -1;
+  int col_block = -1;
   bool no_store = false;
 };
 

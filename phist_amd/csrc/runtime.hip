@@ -87,6 +87,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SPMV_BLOCKS_PER_CU")) r.spmv_blocks_per_cu = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_LANE_BYTES")) r.spmv_lane_bytes = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_DEPTH")) r.spmv_depth = atoi(s);
+  if (const char* s = getenv("PHIST_AMD_SPMV_COL_BLOCK")) r.spmv_col_block = std::max(-1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_SPMV_NT_Y")) r.spmv_nt_y = atoi(s) < 0 ? -1 : (atoi(s) != 0 ? 1 : 0);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
@@ -752,6 +753,14 @@ void phist_amd_set_spmv_depth(int slots, int* iflag) {
     PHX_REQUIRE(slots == 0 || slots == 8 || slots == 16 || slots == 32, PHIST_INVALID_INPUT,
                 "depth %d: 0, 8, 16 or 32", slots);
     rt().spmv_depth = slots == 0 ? -1 : slots;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_spmv_col_block(int ncols, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_REQUIRE(ncols >= -1, PHIST_INVALID_INPUT, "column block %d: -1, 0 or a positive width", ncols);
+    rt().spmv_col_block = ncols;
     *iflag = 0;
   });
 }

@@ -577,7 +577,13 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
       if (ok) VPL = w;
     }
   }
-  const int maxcols = 64 * VPL;
+  // block columns per launch: 64 lane groups, or fewer as set or tuned (each
+  // further launch reads the matrix again; each row's sum is the same)
+  int maxcols = 64 * VPL;
+  int colblock = req.col_block;
+  if (colblock < 0) colblock = rt().spmv_col_block;
+  if (colblock < 0) colblock = tuned != A->tuned.end() ? tuned->second.col_block : 0;
+  if (!gram && colblock > 0 && colblock % VPL == 0) maxcols = std::min(maxcols, colblock);
 
   SpmvArgs<T> a;
   a.chunk_ptr = A->d_chunk_ptr;
@@ -656,23 +662,23 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
                      : VPL == V16 ? launch_xsc<T, V16>(a, lpr, depth)
                      : VPL == 2 * V16 ? launch_xsc<T, 2 * V16>(a, lpr, depth)
                                       : launch_xsc<T, 4 * V16>(a, lpr, depth);
-    if (dots && req.dots_dev) {
-      PHX_REQUIRE(cb == 0 && nc == nvec, PHIST_INVALID_INPUT, "device dots need one column block (%d columns)", nvec);
+    if (dots && req.dots_dev && cb == 0 && nc == nvec) {
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
                          grid, 2 * nc, static_cast<T*>(req.dots_dev));
       PHX_HIP(hipGetLastError());
-    } else if (dots && (req.dot_ww_dev || req.dot_vw_dev)) {
+    } else if (dots && (req.dots_dev || req.dot_ww_dev || req.dot_vw_dev)) {
       // fixed-order sums into the tail of the partials buffer (sized for them), then to the two outputs
+      // (dots_dev: ww at [0, nvec), vw at [nvec, 2 nvec), this block's columns at offset cb)
       T* sums = a.partials + (size_t)grid * 2 * nc;
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
                          grid, 2 * nc, sums);
       PHX_HIP(hipGetLastError());
-      if (req.dot_ww_dev)
-        PHX_HIP(hipMemcpyAsync(static_cast<T*>(req.dot_ww_dev) + cb, sums, sizeof(T) * nc, hipMemcpyDeviceToDevice,
-                               rt().stream));
-      if (req.dot_vw_dev)
-        PHX_HIP(hipMemcpyAsync(static_cast<T*>(req.dot_vw_dev) + cb, sums + nc, sizeof(T) * nc,
-                               hipMemcpyDeviceToDevice, rt().stream));
+      T* ww_out = req.dots_dev ? static_cast<T*>(req.dots_dev) : static_cast<T*>(req.dot_ww_dev);
+      T* vw_out = req.dots_dev ? static_cast<T*>(req.dots_dev) + nvec : static_cast<T*>(req.dot_vw_dev);
+      if (ww_out)
+        PHX_HIP(hipMemcpyAsync(ww_out + cb, sums, sizeof(T) * nc, hipMemcpyDeviceToDevice, rt().stream));
+      if (vw_out)
+        PHX_HIP(hipMemcpyAsync(vw_out + cb, sums + nc, sizeof(T) * nc, hipMemcpyDeviceToDevice, rt().stream));
     } else if (dots) {
       std::vector<T> sums(2 * nc);
       reduce_partials_to_host(a.partials, grid, 2 * nc, sums.data());

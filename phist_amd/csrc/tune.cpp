@@ -12,7 +12,10 @@
 //     footprint of the gathered x rows);
 //   * per block width, for rows of at least 16 entries, the depth: 8, 16 or
 //     32 entries of a row whose index loads and gathers go out together (16
-//     and 32 also prefetch the next batch's indices).
+//     and 32 also prefetch the next batch's indices);
+//   * per block width above 16 columns, launches of 16 (and above 32, of 32)
+//     columns instead of one: each reads the matrix again, but gathers rows
+//     of x half (a quarter) as wide, so fewer of them leave L2.
 // The choice is stored on the matrix (SparseMat::tuned, used by every later
 // SpMV of that width unless a global setting overrides it) and cached for
 // the process by a matrix signature (scalar size, rows, entries, longest row,
@@ -98,12 +101,14 @@ struct TrialBlock {
 // fastest of reps products y = A x after one untimed launch; lb/bpc/depth < 0
 // take the library's usual choice (global settings, tuned entry, table)
 template <typename T>
-double time_product(const SparseMat<T>* A, Mvec<T>* x, Mvec<T>* y, int lb, int bpc, int depth, int reps) {
+double time_product(const SparseMat<T>* A, Mvec<T>* x, Mvec<T>* y, int lb, int bpc, int depth, int reps,
+                    int col_block = -1) {
   Runtime& r = rt();
   SpmvRequest rq;
   rq.lane_bytes = lb;
   rq.blocks_per_cu = bpc;
   rq.depth = depth;
+  rq.col_block = col_block;
   spmv_launch<T>(A, T(1), nullptr, x, T(0), y, rq, nullptr, nullptr);
   double best = 1e300;
   for (int k = 0; k < reps; k++) {
@@ -140,22 +145,31 @@ SpmvTuned search_launch(const SparseMat<T>* A, int nvec, int reps) {
   std::vector<int> depths{8};
   if (A->max_row_len >= 16) depths.push_back(16);
   if (A->max_row_len >= 24) depths.push_back(32);
-  for (int lb : {0, (int)sizeof(T), 32, 64}) {
-    const int vpl = lanes_vpl<T>(nvec, lb);
-    if (std::find(seen.begin(), seen.end(), vpl) != seen.end()) continue;
-    seen.push_back(vpl);
-    std::vector<int> tried;
-    for (int d : depths) {
-      const int depth = spmv_effective_depth(d, vpl, (int)sizeof(T), nvec);
-      if (std::find(tried.begin(), tried.end(), depth) != tried.end()) continue;
-      tried.push_back(depth);
-      for (int bpc : {1, 2, 3, 4, 6, 8}) {
-        const double ms = time_product<T>(A, &x.v, &y.v, lb, bpc, depth, reps);
-        if (ms < best.ms) {
-          best.ms = ms;
-          best.lane_bytes = lb;
-          best.blocks_per_cu = bpc;
-          best.depth = depth;
+  std::vector<int> blocks{0};
+  for (int w : {16, 32})
+    if (nvec > w) blocks.push_back(w);
+  for (int cbw : blocks) {
+    seen.clear();
+    const int ncol = cbw > 0 ? cbw : nvec;  // the widest launch
+    for (int lb : {0, (int)sizeof(T), 32, 64}) {
+      const int vpl = lanes_vpl<T>(nvec, lb);
+      if (std::find(seen.begin(), seen.end(), vpl) != seen.end()) continue;
+      seen.push_back(vpl);
+      if (cbw > 0 && cbw % vpl != 0) continue;
+      std::vector<int> tried;
+      for (int d : depths) {
+        const int depth = spmv_effective_depth(d, vpl, (int)sizeof(T), ncol);
+        if (std::find(tried.begin(), tried.end(), depth) != tried.end()) continue;
+        tried.push_back(depth);
+        for (int bpc : {1, 2, 3, 4, 6, 8}) {
+          const double ms = time_product<T>(A, &x.v, &y.v, lb, bpc, depth, reps, cbw);
+          if (ms < best.ms) {
+            best.ms = ms;
+            best.lane_bytes = lb;
+            best.blocks_per_cu = bpc;
+            best.depth = depth;
+            best.col_block = cbw;
+          }
         }
       }
     }
@@ -165,7 +179,7 @@ SpmvTuned search_launch(const SparseMat<T>* A, int nvec, int reps) {
 
 }  // namespace
 
-// settings[5*i .. 5*i+4] = C, sigma, lane bytes, workgroups per CU, depth for nvecs[i]
+// settings[6*i .. 6*i+5] = C, sigma, lane bytes, workgroups per CU, depth, column block for nvecs[i]
 template <typename T>
 void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, double* default_ms, double* tuned_ms,
                  int* settings) {
@@ -241,11 +255,12 @@ void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, doubl
     A->tuned[nvecs[i]].ms = ms;
     if (tuned_ms) tuned_ms[i] = ms;
     if (settings) {
-      settings[5 * i + 0] = A->C;
-      settings[5 * i + 1] = A->sigma;
-      settings[5 * i + 2] = chosen[i].launch.lane_bytes;
-      settings[5 * i + 3] = chosen[i].launch.blocks_per_cu;
-      settings[5 * i + 4] = chosen[i].launch.depth;
+      settings[6 * i + 0] = A->C;
+      settings[6 * i + 1] = A->sigma;
+      settings[6 * i + 2] = chosen[i].launch.lane_bytes;
+      settings[6 * i + 3] = chosen[i].launch.blocks_per_cu;
+      settings[6 * i + 4] = chosen[i].launch.depth;
+      settings[6 * i + 5] = chosen[i].launch.col_block;
     }
   }
   sync_stream();

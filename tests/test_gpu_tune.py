@@ -21,13 +21,13 @@ def _tune(be, A, nvecs, flags):
     n = len(nvecs)
     nv = np.array(nvecs, dtype=np.int32)
     dms, tms = np.zeros(n), np.zeros(n)
-    st = np.zeros(5 * n, dtype=np.int32)
+    st = np.zeros(6 * n, dtype=np.int32)
     fl = ctypes.c_int(0)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     getattr(be.L, f"phist_{be.p}amd_sparseMat_tune")(A.h, ctypes.c_int(n), p(nv), ctypes.c_int(flags), p(dms),
                                                       p(tms), p(st), ctypes.byref(fl))
     assert fl.value == 0
-    return dms, tms, st.reshape(n, 5)
+    return dms, tms, st.reshape(n, 6)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -49,9 +49,10 @@ def test_tune_layout_and_launch(be64, be32, dtype):
         y.delete()
     dms, tms, st = _tune(be, A, [1, 8], 1 | 2)
     item = np.dtype(dtype).itemsize
-    for (C, sigma, lb, bpc, depth), d, t in zip(st, dms, tms):
+    for (C, sigma, lb, bpc, depth, cblk), d, t in zip(st, dms, tms):
         assert C in (16, 32, 64) and sigma in (1, 256)
         assert lb in (0, item, 32, 64) and bpc in (1, 2, 3, 4, 6, 8) and depth in (8, 16, 32)
+        assert cblk == 0  # widths 1 and 8: one launch
         assert 0 < t <= 1.5 * d + 0.02, (t, d)
     s = A.sell()
     assert s["C"] == st[0][0]
@@ -132,3 +133,75 @@ def test_spmv_depth_bitexact(be64, be32, dtype, nvec, lane_bytes):
         x.delete()
         A.delete()
         m.delete()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [32, 48])
+def test_spmv_column_blocks_bitexact(be64, be32, dtype, nvec):
+    """launches of 16 or 32 block columns (phist_amd_set_spmv_col_block) give the one-launch SpMV
+    result bit for bit, shifts and beta included; fused column dots (to the host and, through the
+    device MINRES, to the device) agree to rounding; the tuner may choose a column block at 48
+    columns and its choice keeps the product bit for bit"""
+    be = be64 if dtype == np.float64 else be32
+    n = 30000
+    A = be.banded_random(n, 20, 2000, 40.0, 55)
+    ro, ci, va = A.crs()
+    ref = ol.Crs(n, n, ro, ci, va.astype(float))
+    rng = np.random.default_rng(nvec)
+    X = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    Y0 = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+    shifts = rng.uniform(-2, 2, nvec).astype(dtype)
+    m = be.map(n)
+    x = be.mvec(m, nvec, X)
+    fl = ctypes.c_int(0)
+    out = {}
+    try:
+        for cb in (0, 16, 32):
+            be.L.phist_amd_set_spmv_col_block(cb, ctypes.byref(fl))
+            assert fl.value == 0
+            y = be.mvec(m, nvec, Y0)
+            be.spmv_shifted(0.7, A, shifts, x, 0.3, y)
+            w = be.mvec(m, nvec, Y0)
+            ww, vw = be.fused_spmv_mvdot(1.1, A, x, 0.0, w)
+            out[cb] = (y.get(), w.get(), ww, vw)
+            y.delete()
+            w.delete()
+    finally:
+        be.L.phist_amd_set_spmv_col_block(-1, ctypes.byref(fl))
+    want = ol.spmv(ref, 0.7, X.astype(float), 0.3, Y0.astype(float), shifts=shifts.astype(float))
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    num = np.linalg.norm(out[0][0].astype(float) - want, axis=0)
+    assert np.all(num <= tol * np.linalg.norm(want, axis=0))
+    for cb in (16, 32):
+        assert np.array_equal(out[cb][0], out[0][0]) and np.array_equal(out[cb][1], out[0][1]), cb
+        for g, d in zip(out[cb][2:], out[0][2:]):
+            assert np.max(np.abs(g - d) / np.abs(d)) <= (1e-13 if dtype == np.float64 else 1e-5)
+    # the device dots of the MINRES iteration with column blocks: the same solutions to the tolerance
+    if dtype == np.float64:  # (MINRES needs a symmetric matrix: a 3D Laplacian)
+        Al = be.laplace3d(30, 30, 30)
+        ml = be.map(27000)
+        B = be.mvec(ml, nvec, rng.uniform(-1, 1, (27000, nvec)))
+        sol = {}
+        try:
+            for cb in (0, 16):
+                be.L.phist_amd_set_spmv_col_block(cb, ctypes.byref(fl))
+                Xs = be.mvec(ml, nvec)
+                it, rr, unconv = be.minres_shifted(Al, np.full(nvec, 0.5), B, Xs, 1e-10, 500)
+                assert unconv == 0
+                sol[cb] = Xs.get()
+                Xs.delete()
+        finally:
+            be.L.phist_amd_set_spmv_col_block(-1, ctypes.byref(fl))
+        assert np.max(np.abs(sol[16] - sol[0])) <= 1e-8 * np.max(np.abs(sol[0]))
+        for o in (B, Al, ml):
+            o.delete()
+    # the tuner's choice at this width
+    before = out[0][1]
+    _tune(be, A, [nvec], 2)
+    w = be.mvec(m, nvec, Y0)
+    be.fused_spmv_mvdot(1.1, A, x, 0.0, w)
+    assert np.array_equal(w.get(), before)
+    for o in (x, w, A):
+        o.delete()
+    m.delete()
+
