@@ -18,7 +18,9 @@ events on the library stream during the timed steps.
 cpu_baseline: PHIST's own builtin leaf kernels (dspmvm_8 + two ddot_8,
 compiled from the reference sources by oracle/Makefile, OpenMP) on the
 first quarter of the same matrix, timed on this host; if that build is
-absent, the oracle's single-threaded C restatement.
+absent, the oracle's single-threaded C restatement.  Beside it the host's
+STREAM triad (z = x + a*y, 24 bytes per element, all threads) as the host's
+own bandwidth ceiling.
 """
 import argparse
 import ctypes
@@ -145,8 +147,15 @@ def cpu_baseline(A, n_sample, nvec, budget_s=12.0):
     gbs = spmv_bytes(nnz_s, n_sample, nvec) / t / 1e9
     sample = (f"first {n_sample} rows ({nnz_s} nnz) of the same matrix, nvec={nvec}, "
               f"spmv (alpha=1, beta=0) + 2 column dots, median of {len(times)} runs")
+    del x, y
+    # the host's own memory bandwidth beside it: STREAM triad on 3 x 1 GiB
+    # arrays, all OpenMP threads (what the reference's stream benchmark reports)
+    triad = ol.host_stream_triad(1 << 27, 10)
     return {"value": round(gbs, 2), "unit": "GB/s", "cores": threads, "kind": kind, "sample": sample,
-            "ms_per_step": round(t * 1e3, 3), **cpu_info_fields(threads)}
+            "ms_per_step": round(t * 1e3, 3),
+            **({} if triad is None else {"host_stream_triad_GBs": round(triad[0], 1),
+                                         "frac_of_host_triad": round(gbs / triad[0], 3)}),
+            **cpu_info_fields(threads)}
 
 
 def tall_skinny(be, args):

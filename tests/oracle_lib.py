@@ -11,6 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libbuiltin_leaf.so")
+STREAM_SO = os.path.join(ORACLE_DIR, "libhost_stream.so")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 MATRICES = os.path.join(GOLDEN, "matrices")
 
@@ -47,6 +48,20 @@ def ref_leaf():
     if not os.path.exists(REF_SO):
         return None
     return ctypes.CDLL(REF_SO)
+
+
+def host_stream_triad(n=1 << 27, reps=10):
+    """the host's STREAM triad z = x + a*y over n doubles on the OpenMP threads
+    (oracle/host_stream.c): (best, mean) GB/s of reps runs at 24 bytes per
+    element, or None if that library is not built"""
+    if not os.path.exists(STREAM_SO):
+        return None
+    lib = ctypes.CDLL(STREAM_SO)
+    best, mean = ctypes.c_double(), ctypes.c_double()
+    rc = lib.host_stream_triad(ctypes.c_int64(n), ctypes.c_int(reps), ctypes.byref(best), ctypes.byref(mean))
+    if rc != 0:
+        raise RuntimeError(f"host_stream_triad returned {rc}")
+    return best.value, mean.value
 
 
 class Crs:
