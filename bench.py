@@ -309,13 +309,13 @@ def jada_bench(be, args):
         # matrix and on its FP32 copy (the correction solves of the mixed run)
         pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         nv, dms, tms, st = (np.array([cfg["block_size"]], np.int32), np.zeros(1), np.zeros(1),
-                            np.zeros(6, np.int32))
+                            np.zeros(7, np.int32))
         iflag = ctypes.c_int(0)
         t0 = time.perf_counter()
         be.L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 4, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
         if iflag.value:
             raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
-        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "col_block": int(st[5]),
+        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "col_block": int(st[5]), "stage": int(st[6]),
                 "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
                 "tune_s": round(time.perf_counter() - t0, 2)}
         print(f"bench: autotuner (FP64 matrix; its FP32 copy tuned too) {tune}", file=sys.stderr, flush=True)
@@ -408,8 +408,8 @@ def jada_cpu_baseline(A, cfg, inner):
 def tune_bench(be, args):
     """The SpMV autotuner on the matrices of the workloads: per matrix and block width the kernel time per
     product with the library's untuned choice (default SELL layout, launch table) and with the settings
-    the tuner picks over SELL layouts (C 16/32/64, sigma 1/256), gather width per lane and workgroups
-    per CU; one JSON line per matrix."""
+    the tuner picks over SELL layouts (C 16/32/64, sigma 1/256), gather width per lane, workgroups
+    per CU, depth, column blocks and LDS staging of x; one JSON line per matrix."""
     L = be.L
     pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     cases = [
@@ -417,7 +417,8 @@ def tune_bench(be, args):
          [1, 4, 8, 16, 32]),
         ("XXZ chain L=24 S_z=0, FP64", lambda: be.spinchain_sz(24, 12), [4, 16]),
         ("honeycomb 2000x4000 sites (8M rows), FP64", lambda: be.honeycomb(2000, 4000, 0, 0, 0, 1.0, 77), [32]),
-        ("3D Laplacian 128^3, FP64", lambda: be.laplace3d(128, 128, 128), [1, 4, 16]),
+        ("3D Laplacian 128^3, FP64", lambda: be.laplace3d(128, 128, 128), [1, 4, 8, 16]),
+        ("3D Laplacian 64^3, FP64", lambda: be.laplace3d(64, 64, 64), [4]),
     ]
     iflag = ctypes.c_int(0)
     L.phist_amd_sparseMat_set_format(0, 0, ctypes.byref(iflag))
@@ -426,7 +427,7 @@ def tune_bench(be, args):
         A = make()
         t_build = time.perf_counter() - t0
         n = len(widths)
-        nv, dms, tms, st = np.array(widths, np.int32), np.zeros(n), np.zeros(n), np.zeros(6 * n, np.int32)
+        nv, dms, tms, st = np.array(widths, np.int32), np.zeros(n), np.zeros(n), np.zeros(7 * n, np.int32)
         t0 = time.perf_counter()
         L.phist_Damd_sparseMat_tune(A.h, n, pp(nv), 1 | 2, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
         if iflag.value:
@@ -434,16 +435,65 @@ def tune_bench(be, args):
         t_tune = time.perf_counter() - t0
         rows = []
         for i, w in enumerate(widths):
-            C, sg, lb, bpc, depth, cblk = (int(v) for v in st[6 * i:6 * i + 6])
+            C, sg, lb, bpc, depth, cblk, stg = (int(v) for v in st[7 * i:7 * i + 7])
             gb = spmv_bytes(A.nnz, A.nrows, w) / 1e9
+            # the product with x staged in LDS (the tuned launch otherwise; best of the workgroups per CU),
+            # or None where no tile window of this width fits
+            staged = staged_product_ms(be, A, w, lb, depth, cblk) if not stg else None
             rows.append({"nvec": w, "untuned_ms": round(float(dms[i]), 4), "tuned_ms": round(float(tms[i]), 4),
                          "gain": round(float(dms[i] / tms[i]), 3), "C": C, "sigma": sg, "lane_bytes": lb,
-                         "blocks_per_cu": bpc, "depth": depth, "col_block": cblk, "tuned_GBs": round(gb / (tms[i] * 1e-3), 1)})
+                         "blocks_per_cu": bpc, "depth": depth, "col_block": cblk, "stage": stg,
+                         "staged_ms": None if staged is None else round(staged, 4),
+                         "tuned_GBs": round(gb / (tms[i] * 1e-3), 1)})
         out = {"metric": "SpMV autotuner: kernel ms per product, untuned vs tuned", "workload": name,
                "rows": A.nrows, "nnz": A.nnz, "build_s": round(t_build, 2), "tune_s": round(t_tune, 2),
                "widths": rows}
         print(json.dumps(out), flush=True)
         A.delete()
+
+
+def staged_product_ms(be, A, nvec, lane_bytes, depth, col_block, reps=20):
+    """kernel ms per product y = A x with x staged in LDS per workgroup tile (phist_amd_set_spmv_stage),
+    lane bytes, depth and column block as given, the fastest over 1-8 workgroups per CU; None if the
+    launch does not stage (no tile window of this width fits)"""
+    L, fl = be.L, ctypes.c_int(0)
+    m = be.map(A.nrows)
+    x, y = be.mvec(m, nvec), be.mvec(m, nvec)
+    x.random()
+    best = None
+    try:
+        L.phist_amd_set_spmv_stage(1, ctypes.byref(fl))
+        L.phist_amd_set_spmv_lane_bytes(lane_bytes, ctypes.byref(fl))
+        L.phist_amd_set_spmv_depth(depth, ctypes.byref(fl))
+        L.phist_amd_set_spmv_col_block(col_block, ctypes.byref(fl))
+        for bpc in (1, 2, 3, 4, 6, 8):
+            L.phist_amd_set_spmv_blocks_per_cu(bpc, ctypes.byref(fl))
+            c0, c1 = ctypes.c_int64(0), ctypes.c_int64(0)
+            L.phist_amd_spmv_staged_launches(ctypes.byref(c0), ctypes.byref(fl))
+            be.spmv(1.0, A, x, 0.0, y)
+            L.phist_amd_spmv_staged_launches(ctypes.byref(c1), ctypes.byref(fl))
+            if c1.value == c0.value:
+                return None
+            be.synchronize()
+            L.phist_amd_kernel_timer(1, ctypes.byref(fl))
+            for _ in range(reps):
+                be.spmv(1.0, A, x, 0.0, y)
+            be.synchronize()
+            kms, kc = ctypes.c_double(0.0), ctypes.c_int64(0)
+            L.phist_amd_kernel_timer_read(ctypes.byref(kms), ctypes.byref(kc), ctypes.byref(fl))
+            L.phist_amd_kernel_timer(0, ctypes.byref(fl))
+            ms = kms.value / reps  # per product (a column-blocked product is several launches)
+            best = ms if best is None else min(best, ms)
+    finally:
+        L.phist_amd_set_spmv_stage(-1, ctypes.byref(fl))
+        L.phist_amd_set_spmv_blocks_per_cu(-1, ctypes.byref(fl))
+        L.phist_amd_set_spmv_lane_bytes(0, ctypes.byref(fl))
+        L.phist_amd_set_spmv_depth(0, ctypes.byref(fl))
+        L.phist_amd_set_spmv_col_block(-1, ctypes.byref(fl))
+        x.delete()
+        y.delete()
+        m.delete()
+    return best
 
 
 def unit_random_block(be, m, k):
@@ -486,14 +536,14 @@ def minres_bench(be, args, W=2000, L=4000, nsys=32, tol=1e-6, max_iter=3000):
     # the library's autotuner picks the SELL layout and launch settings of the
     # 32-column product for the matrix and its FP32 copy (set-up, not timed)
     pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    nv, dms, tms, st = np.array([nsys], np.int32), np.zeros(1), np.zeros(1), np.zeros(6, np.int32)
+    nv, dms, tms, st = np.array([nsys], np.int32), np.zeros(1), np.zeros(1), np.zeros(7, np.int32)
     iflag = ctypes.c_int(0)
     t0 = time.perf_counter()
     be.L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 1 | 4, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
     if iflag.value:
         raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
     tune = {"C": int(st[0]), "sigma": int(st[1]), "lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]),
-            "depth": int(st[4]), "col_block": int(st[5]), "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
+            "depth": int(st[4]), "col_block": int(st[5]), "stage": int(st[6]), "untuned_spmv_ms": round(float(dms[0]), 4), "tuned_spmv_ms": round(float(tms[0]), 4),
             "tune_s": round(time.perf_counter() - t0, 2)}
     print(f"bench: autotuner {tune}", file=sys.stderr, flush=True)
     m = A.domain_map
@@ -1008,13 +1058,13 @@ def main():
     if not args.no_tune and args.lane_bytes == 0 and args.blocks_per_cu < 0 and args.depth == 0 and args.col_block < 0:
         # the library's autotuner picks the launch settings for this width; the
         # layout stays the configuration's SELL-32-256
-        nv, dms, tms, st = np.array([nvec], np.int32), np.zeros(1), np.zeros(1), np.zeros(6, np.int32)
+        nv, dms, tms, st = np.array([nvec], np.int32), np.zeros(1), np.zeros(1), np.zeros(7, np.int32)
         pp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         t0 = time.perf_counter()
         L.phist_Damd_sparseMat_tune(A.h, 1, pp(nv), 0, pp(dms), pp(tms), pp(st), ctypes.byref(iflag))
         if iflag.value:
             raise RuntimeError(f"phist_Damd_sparseMat_tune iflag={iflag.value}")
-        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "col_block": int(st[5]),
+        tune = {"lane_bytes": int(st[2]), "blocks_per_cu": int(st[3]), "depth": int(st[4]), "col_block": int(st[5]), "stage": int(st[6]),
                 "untuned_kernel_ms": round(float(dms[0]), 4),
                 "tuned_kernel_ms": round(float(tms[0]), 4), "tune_s": round(time.perf_counter() - t0, 2)}
         print(f"bench: autotuner {tune}", file=sys.stderr, flush=True)

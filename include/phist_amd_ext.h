@@ -50,18 +50,31 @@ void phist_amd_set_spmv_depth(int slots, int* iflag);
  * columns) gather half as many bytes per launch in two blocks of 16.  Every
  * choice gives the same result bit for bit (column dots to rounding). */
 void phist_amd_set_spmv_col_block(int ncols, int* iflag);
+/* x staged in LDS: mode 1 = each workgroup first copies the x rows its tile
+ * of SELL rows gathers from (the tile's column intervals, found once per
+ * matrix and tile height) into LDS with 16-byte loads, then gathers from
+ * there; used where every tile's window fits the workgroup's share of LDS
+ * (stencil-like matrices: a 3D Laplacian tile gathers from three short
+ * intervals) and the block rows are whole 16-byte vectors (at most 8 lanes
+ * per row, no per-column x factors, no Gram epilogue), else ignored.  0 = off,
+ * -1 = the default: the matrix's tuned value, else off.  Every choice gives
+ * the same result bit for bit. */
+void phist_amd_set_spmv_stage(int mode, int* iflag);
+/* SpMV launches so far (this process) that staged x in LDS */
+void phist_amd_spmv_staged_launches(int64_t* count, int* iflag);
 /* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16
  * widths) it times the product y = A*x on the device over the gather width
  * per lane (one scalar, or one, two or four 16-byte vectors: how many lanes
  * share a row), the workgroups per CU (1, 2, 3, 4, 6, 8), for rows of at
  * least 16 entries the depth (8, 16 or 32, phist_amd_set_spmv_depth), for more
  * than 16 columns launches of 16 (and of 32 above 32) columns
- * (phist_amd_set_spmv_col_block), and with
+ * (phist_amd_set_spmv_col_block), where the matrix allows it x staged in LDS
+ * per workgroup tile (phist_amd_set_spmv_stage), and with
  * flags & 1 also over the SELL-C-sigma layout (C = 16, 32, 64; sigma = 1,
  * 256; the device copy is rebuilt from the host CRS for each).  The best
  * layout (least total time over the widths) is kept, and the best launch
  * settings of each width are stored on the matrix: later SpMVs of that width
- * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu / _depth / _col_block set global
+ * use them unless phist_amd_set_spmv_lane_bytes / _blocks_per_cu / _depth / _col_block / _stage set global
  * values.  Choices are cached for the process by matrix signature (scalar
  * type, rows, entries, longest row, mean distance of entries from the
  * diagonal, block width); flags & 2 measures again instead of using the
@@ -69,7 +82,8 @@ void phist_amd_set_spmv_col_block(int ncols, int* iflag);
  * mixed-precision solvers (launch settings, same widths).  Outputs may be
  * NULL: default_ms[i] / tuned_ms[i] = fastest of three products with the
  * library's untuned choice on the incoming layout / with the chosen settings;
- * settings[6*i..6*i+5] = C, sigma, lane bytes, workgroups per CU, depth, column block.  Products
+ * settings[7*i..7*i+6] = C, sigma, lane bytes, workgroups per CU, depth, column block,
+ * staging (1 = x staged in LDS).  Products
  * are identical bit for bit under every setting. */
 void phist_Damd_sparseMat_tune(phist_DsparseMat_ptr A, int n_nvec, const int* nvecs, int flags, double* default_ms,
                                double* tuned_ms, int* settings, int* iflag);

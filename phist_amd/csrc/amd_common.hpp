@@ -73,6 +73,7 @@ struct Runtime {
   bool initialized = false;
   int device = 0;
   int num_cu = 256;
+  int64_t lds_per_block = 65536;  // LDS a workgroup may allocate (static + dynamic)
   int num_xcd = 8;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -110,6 +111,11 @@ struct Runtime {
   // block columns per SpMV launch: > 0 at most that many, 0 as many as 64
   // lane groups hold, -1 (default) the matrix's tuned value
   int spmv_col_block = -1;
+  // x rows of each workgroup tile staged in LDS before the tile's gathers
+  // (matrices whose tiles gather from a few short column intervals): 1 on
+  // where the matrix allows it, 0 off, -1 (default) the matrix's tuned value
+  int spmv_stage = -1;
+  int64_t spmv_staged_launches = 0;  // SpMV launches that staged x in LDS (a statistic)
   static constexpr int64_t kNtYBytes = int64_t(64) << 20;
   // Gram products with both operands >= 16 columns run on the matrix cores
   bool gram_mfma = true;
@@ -291,7 +297,25 @@ struct SpmvTuned {
   int blocks_per_cu = 0;
   int depth = 8;  // entries per row whose loads are issued together (8 or 32)
   int col_block = 0;  // block columns per launch (0: as many as 64 lane groups hold)
+  int stage = 0;      // x rows of each tile staged in LDS
   double ms = 0.0;  // measured time per product when chosen
+};
+
+// The x rows a workgroup tile of rpb consecutive SELL rows gathers from, as
+// column intervals, for the SpMV that stages them in LDS (spmv.hip): tile t
+// owns intervals [ivptr[t], ivptr[t+1]), interval k copies x rows
+// iv[3k] .. iv[3k] + iv[3k+2] - 1 to window rows iv[3k+1] ..; lcol[slot] is
+// the window row of the column of SELL slot `slot` (padding included).  ok
+// when every tile needs at most kMaxIv intervals and kMaxRows window rows.
+struct SellStage {
+  static constexpr int kMaxIv = 64;
+  static constexpr int kMaxRows = 65535;
+  bool ok = false;
+  int max_rows = 0;  // the widest window over the tiles
+  int max_iv = 0;    // the most intervals of a tile
+  int32_t* d_ivptr = nullptr;
+  int32_t* d_iv = nullptr;
+  uint16_t* d_lcol = nullptr;
 };
 
 template <typename T>
@@ -330,7 +354,22 @@ struct SparseMat {
   // SpMV launch settings the autotuner chose for this matrix, by block width
   // (tune.cpp); the SpMV uses them unless a global setting overrides them
   std::map<int, SpmvTuned> tuned;
+  // LDS staging tables by tile height (sell_stage; made on first use, freed
+  // with the SELL copy)
+  mutable std::map<int, SellStage> stage;
 };
+
+// the staging tables of A for tiles of rpb SELL rows (built on first use:
+// host work over every slot, then one upload); nullptr when some tile
+// gathers from more intervals or window rows than SellStage allows
+template <typename T> const SellStage* sell_stage(const SparseMat<T>* A, int rpb);
+// LDS a staged SpMV window may take per workgroup (two workgroups per CU still fit)
+constexpr int kStageLdsBytes = 64 << 10;
+// the staging tables for an SpMV launch of nc columns with vpl scalars per
+// lane (spmv.hip), with log2 of the 16-byte pieces per window row and the
+// window's LDS bytes; nullptr where that launch cannot stage
+template <typename T>
+const SellStage* spmv_stage_for(const SparseMat<T>* A, int nc, int vpl, int* pshift, size_t* lds_bytes);
 
 // the cached FP32 copy of A (same SELL layout, values rounded to float)
 const SparseMat<float>* sparse_fp32_twin(const SparseMat<double>* A);
@@ -400,6 +439,7 @@ struct SpmvRequest {
   int blocks_per_cu = -1;
   int depth = -1;
   int col_block = -1;
+  int stage = -1;
   bool no_store = false;
 };
 

@@ -76,6 +76,7 @@ void ensure_init() {
   hipDeviceProp_t prop;
   PHX_HIP(hipGetDeviceProperties(&prop, r.device));
   r.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (prop.sharedMemPerBlock > 0) r.lds_per_block = (int64_t)prop.sharedMemPerBlock;
   PHX_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
   PHX_HIP(hipEventCreate(&r.ev_start));
   PHX_HIP(hipEventCreate(&r.ev_stop));
@@ -88,6 +89,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SPMV_LANE_BYTES")) r.spmv_lane_bytes = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_DEPTH")) r.spmv_depth = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_COL_BLOCK")) r.spmv_col_block = std::max(-1, atoi(s));
+  if (const char* s = getenv("PHIST_AMD_SPMV_STAGE")) r.spmv_stage = std::max(-1, std::min(1, atoi(s)));
   if (const char* s = getenv("PHIST_AMD_SPMV_NT_Y")) r.spmv_nt_y = atoi(s) < 0 ? -1 : (atoi(s) != 0 ? 1 : 0);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
@@ -761,6 +763,22 @@ void phist_amd_set_spmv_col_block(int ncols, int* iflag) {
     ensure_init();
     PHX_REQUIRE(ncols >= -1, PHIST_INVALID_INPUT, "column block %d: -1, 0 or a positive width", ncols);
     rt().spmv_col_block = ncols;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_spmv_stage(int mode, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_REQUIRE(mode >= -1 && mode <= 1, PHIST_INVALID_INPUT, "staging mode %d: -1, 0 or 1", mode);
+    rt().spmv_stage = mode;
+    *iflag = 0;
+  });
+}
+void phist_amd_spmv_staged_launches(int64_t* count, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    PHX_REQUIRE(count, PHIST_INVALID_INPUT, "NULL count");
+    *count = rt().spmv_staged_launches;
     *iflag = 0;
   });
 }

@@ -16,8 +16,14 @@
 //    row's own index (or 0 if that is not a valid column), so they never
 //    address outside x.  The row permutation stays inside the matrix: block
 //    vectors keep their natural order, y is written through perm.
+// 3. LDS staging tables (sell_stage): for tiles of consecutive SELL rows, the
+//    sorted columns the tile gathers from (padding slots included), merged
+//    into intervals across gaps of at most kStageGap rows, and each slot's
+//    row in the tile's window of intervals laid end to end.
 #include "amd_common.hpp"
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <numeric>
 #include <omp.h>
 
@@ -180,9 +186,126 @@ void sell_build(SparseMat<T>* A, int C, int sigma) {
   A->h_perm.swap(perm);
 }
 
+namespace {
+// unused rows between two gathered columns that one interval still spans
+constexpr int32_t kStageGap = 4;
+
+template <typename T>
+void stage_free(const SparseMat<T>* A) {
+  for (auto& kv : A->stage) {
+    for (void* p : {(void*)kv.second.d_ivptr, (void*)kv.second.d_iv, (void*)kv.second.d_lcol})
+      if (p) PHX_HIP(hipFree(p));
+  }
+  A->stage.clear();
+}
+}  // namespace
+
+template <typename T>
+const SellStage* sell_stage(const SparseMat<T>* A, int rpb) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  auto found = A->stage.find(rpb);
+  if (found != A->stage.end()) return found->second.ok ? &found->second : nullptr;
+  SellStage& S = A->stage[rpb];  // not ok until the tables are up
+  const int64_t nslots = A->nchunks * A->C;
+  const int64_t ntiles = rpb > 0 ? (nslots + rpb - 1) / rpb : 0;
+  if (ntiles == 0 || A->nnz_padded == 0 || A->crs_ptr.empty() || ntiles >= INT32_MAX) return nullptr;
+  const int C = A->C;
+  const int64_t* cp = A->h_chunk_ptr.data();
+  const int32_t* cw = A->h_chunk_w.data();
+  const int32_t* perm = A->h_perm.data();
+  const int64_t* rp = A->crs_ptr.data();
+  const int32_t* ci = A->crs_col.data();
+  std::vector<std::vector<int32_t>> tiv(ntiles);
+  std::vector<uint16_t> lcol(A->nnz_padded, 0);
+  std::atomic<bool> fail{false};
+  int max_rows = 0, max_iv = 0;
+#pragma omp parallel
+  {
+    std::vector<int32_t> cols;
+    int my_rows = 0, my_iv = 0;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t t = 0; t < ntiles; t++) {
+      if (fail.load(std::memory_order_relaxed)) continue;
+      const int64_t s0 = t * rpb, s1 = std::min(nslots, s0 + rpb);
+      // the column of entry j of SELL row s (the row's own index, or 0, for padding)
+      auto col_of = [&](int64_t s, int32_t j) -> int32_t {
+        const int32_t row = perm[s];
+        const int64_t len = row >= 0 ? rp[row + 1] - rp[row] : 0;
+        if (j < len) return ci[rp[row] + j];
+        return (row >= 0 && row < A->ncols) ? row : 0;
+      };
+      cols.clear();
+      for (int64_t sr = s0; sr < s1; sr++) {
+        const int32_t w = cw[sr / C];
+        for (int32_t j = 0; j < w; j++) cols.push_back(col_of(sr, j));
+      }
+      std::sort(cols.begin(), cols.end());
+      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+      std::vector<int32_t>& iv = tiv[t];  // (source column, window row, length) per interval
+      int32_t rows = 0;
+      for (int32_t c : cols) {
+        if (!iv.empty() && c <= iv[iv.size() - 3] + iv.back() - 1 + kStageGap) {
+          const int32_t grow = c - (iv[iv.size() - 3] + iv.back() - 1);
+          iv.back() += grow;
+          rows += grow;
+        } else {
+          iv.insert(iv.end(), {c, rows, 1});
+          rows += 1;
+        }
+        if (rows > SellStage::kMaxRows || (int)iv.size() / 3 > SellStage::kMaxIv) break;
+      }
+      const int niv = (int)iv.size() / 3;
+      if (rows > SellStage::kMaxRows || niv > SellStage::kMaxIv) {
+        fail.store(true, std::memory_order_relaxed);
+        continue;
+      }
+      my_rows = std::max(my_rows, rows);
+      my_iv = std::max(my_iv, niv);
+      for (int64_t sr = s0; sr < s1; sr++) {
+        const int64_t ch = sr / C;
+        const int r = (int)(sr % C);
+        for (int32_t j = 0; j < cw[ch]; j++) {
+          const int32_t c = col_of(sr, j);
+          int lo = 0, hi = niv - 1;  // the last interval starting at or before c
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (iv[3 * mid] <= c) lo = mid; else hi = mid - 1;
+          }
+          lcol[cp[ch] + (int64_t)j * C + r] = (uint16_t)(iv[3 * lo + 1] + (c - iv[3 * lo]));
+        }
+      }
+    }
+#pragma omp critical
+    {
+      max_rows = std::max(max_rows, my_rows);
+      max_iv = std::max(max_iv, my_iv);
+    }
+  }
+  if (fail.load()) return nullptr;
+  std::vector<int32_t> ivptr(ntiles + 1, 0), ivs;
+  for (int64_t t = 0; t < ntiles; t++) ivptr[t + 1] = ivptr[t] + (int32_t)(tiv[t].size() / 3);
+  ivs.reserve((size_t)ivptr[ntiles] * 3);
+  for (auto& v : tiv) ivs.insert(ivs.end(), v.begin(), v.end());
+  hipStream_t st = rt().stream;
+  PHX_HIP(hipMalloc(&S.d_ivptr, ivptr.size() * sizeof(int32_t)));
+  PHX_HIP(hipMalloc(&S.d_iv, std::max<size_t>(1, ivs.size()) * sizeof(int32_t)));
+  PHX_HIP(hipMalloc(&S.d_lcol, lcol.size() * sizeof(uint16_t)));
+  PHX_HIP(hipMemcpyAsync(S.d_ivptr, ivptr.data(), ivptr.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (!ivs.empty())
+    PHX_HIP(hipMemcpyAsync(S.d_iv, ivs.data(), ivs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  PHX_HIP(hipMemcpyAsync(S.d_lcol, lcol.data(), lcol.size() * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+  PHX_HIP(hipStreamSynchronize(st));  // host vectors die here
+  S.max_rows = max_rows;
+  S.max_iv = max_iv;
+  S.ok = true;
+  return &S;
+}
+
 template <typename T>
 void sell_rebuild(SparseMat<T>* A, int C, int sigma) {
   sync_stream();
+  stage_free(A);
   if (A->fp32_twin) {  // made again from the new layout on first use
     sparse_free(A->fp32_twin);
     A->fp32_twin = nullptr;
@@ -208,6 +331,7 @@ void sparse_free(SparseMat<T>* A) {
   if (A->d_perm) PHX_HIP(hipFree(A->d_perm));
   if (A->d_col) PHX_HIP(hipFree(A->d_col));
   if (A->d_val) PHX_HIP(hipFree(A->d_val));
+  stage_free(A);
   if (A->col_map && A->col_map != A->row_map) delete A->col_map;
   if (A->domain_map && A->domain_map != A->row_map && A->domain_map != A->col_map) delete A->domain_map;
   delete A->row_map;
@@ -400,6 +524,8 @@ SparseMat<T>* sparse_from_mm(const Comm* comm, const char* filename) {
 
 template void sell_build<double>(SparseMat<double>*, int, int);
 template void sell_build<float>(SparseMat<float>*, int, int);
+template const SellStage* sell_stage<double>(const SparseMat<double>*, int);
+template const SellStage* sell_stage<float>(const SparseMat<float>*, int);
 template void sell_rebuild<double>(SparseMat<double>*, int, int);
 template void sell_rebuild<float>(SparseMat<float>*, int, int);
 template void sparse_free<double>(SparseMat<double>*);

@@ -49,6 +49,14 @@ struct SpmvArgs {
   const T* __restrict__ shift_dev;  // shifts on the device (this launch's columns), else shift[]
   const double* __restrict__ xscale;  // per-column factors of x (XSC kernels), stride xscale_stride
   int64_t xscale_stride;
+  // staged kernels: the tile's column intervals and each slot's window row
+  // (SellStage), 16-byte pieces per window row (st_ppr) and log2 of the
+  // pieces a window row occupies in LDS (st_pshift, a power of two >= st_ppr)
+  const int32_t* __restrict__ st_ivptr;
+  const int32_t* __restrict__ st_iv;
+  const uint16_t* __restrict__ st_lcol;
+  int st_ppr, st_pshift;
+  size_t st_lds_bytes;  // host side: dynamic LDS of the staged window
   T shift[64 * 4];
 };
 
@@ -136,6 +144,12 @@ __device__ inline T group_bcast_t(T v, int src, int lane) {
 // gathers, go out before the first FMA of the batch, so a row of 32 entries
 // costs one round trip for its indices and one for its gathers.
 //
+// STG: before a tile's rows, the workgroup copies the x rows the tile
+// gathers from (its column intervals, each taken by one wave with 16-byte
+// loads) into LDS, and the gathers read that window at each slot's window
+// row (st_lcol, 16 bits, read instead of the 32-bit column).  The values
+// gathered and their order are the same, so are the results.
+//
 // GR > 0 (fused_spmv_mvTmv, at most 32 columns, GR = 16-column tiles): after
 // each tile the wave writes its rows of the new W (and of V) to LDS and
 // reads them back in the matrix-core operand layout, four rows per MFMA
@@ -143,8 +157,9 @@ __device__ inline T group_bcast_t(T v, int src, int lane) {
 // registers; the wave tiles are added in LDS in wave order and the
 // workgroup tiles by a second fixed-order pass, so the sums are
 // deterministic.  W itself is not stored with F_NOSTORE (W = NULL).
-template <typename T, int VPL, int LPR, bool XSC, int GR, int SL>
+template <typename T, int VPL, int LPR, bool XSC, int GR, int SL, bool STG = false>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
+  static_assert(!STG || (VPL * sizeof(T) >= 16 && !XSC && GR == 0), "staged windows hold whole 16-byte vectors");
   constexpr int RPW = 64 / LPR;
   constexpr int RPB = (kBlock / 64) * RPW;
   constexpr int KB = SL > LPR ? SL / LPR : 1;  // val/col loads per lane and batch
@@ -181,8 +196,43 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 #pragma unroll
     for (int j = 0; j < TG; j++) gww[i][j] = gvw[i][j] = typename GM::acc_t{0, 0, 0, 0};
 
+  extern __shared__ __align__(16) unsigned char spmv_window[];
+  const T* win_x = reinterpret_cast<const T*>(spmv_window) + c0;  // this lane's columns of window row 0
+  const int win_ld = (16 / (int)sizeof(T)) << a.st_pshift;        // scalars per window row
+
   const TileWalk tw = xcd_tiles(a.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
+    if constexpr (STG) {
+      // the tile's window: wave w copies intervals w, w+4, ...; piece p of an
+      // interval is piece p & (2^pshift - 1) of its row p >> pshift
+      __syncthreads();  // every wave is done with the previous tile's window
+      const int k0 = a.st_ivptr[t], k1 = a.st_ivptr[t + 1];
+      const int ps = a.st_pshift, ppr = a.st_ppr, pmask = (1 << ps) - 1;
+      typedef unsigned int piece_t __attribute__((ext_vector_type(4)));
+      piece_t* win = reinterpret_cast<piece_t*>(spmv_window);
+      for (int k = k0 + wv; k < k1; k += kBlock / 64) {
+        const int32_t src = a.st_iv[3 * k], off = a.st_iv[3 * k + 1], len = a.st_iv[3 * k + 2];
+        const int n = len << ps;
+        const unsigned char* xs = reinterpret_cast<const unsigned char*>(a.x + (int64_t)src * a.ldx);
+        const int64_t ldb = a.ldx * (int64_t)sizeof(T);
+        for (int p0 = 0; p0 < n; p0 += 64 * 8) {
+          piece_t v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int p = p0 + u * 64 + lane;
+            v[u] = piece_t{0, 0, 0, 0};
+            if (p < n && (p & pmask) < ppr)
+              v[u] = *reinterpret_cast<const piece_t*>(xs + (int64_t)(p >> ps) * ldb + 16 * (p & pmask));
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int p = p0 + u * 64 + lane;
+            if (p < n && (p & pmask) < ppr) win[(off << ps) + p] = v[u];
+          }
+        }
+      }
+      __syncthreads();
+    }
     // all lanes stay in the loop body (group broadcasts need every lane)
     const int64_t s = t * RPB + wv * RPW + rw;
     const bool sok = s < a.nslots;
@@ -205,6 +255,20 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < VPL; e++) acc[e] = (flags & F_SHIFT) ? sh[e] * xo[e] : T(0);
 
+    // x row cc of this lane's columns: from the window (STG: cc is a window row) or from x
+    auto gather = [&](int32_t cc, T (&out)[VPL]) {
+      if constexpr (STG)
+        vload<T, VPL>(win_x + cc * win_ld, out);
+      else
+        vload<T, VPL>(xb + (int64_t)cc * a.ldx, out);
+    };
+    // the column (STG: window row) of a slot
+    auto load_col = [&](int64_t slot) -> int32_t {
+      if constexpr (STG)
+        return (int32_t)a.st_lcol[slot];
+      else
+        return __builtin_nontemporal_load(a.col + slot);
+    };
     // the batch's x gathers, then its FMAs in entry order: slot j0+k of this
     // row sits in lane k % LPR of the group, register k / LPR
     auto spmv_batch = [&](int j0, const T (&v)[KB], const int32_t (&cj)[KB]) {
@@ -219,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
           const int32_t cc = group_bcast<LPR>(cj[k / LPR], k % LPR, lane);
           vv[kk] = group_bcast_t<LPR>(v[k / LPR], k % LPR, lane);
           ok[kk] = colok && (j0 + k) < w;
-          if (ok[kk]) vload<T, VPL>(xb + (int64_t)cc * a.ldx, xg[kk]);
+          if (ok[kk]) gather(cc, xg[kk]);
         }
 #pragma unroll
         for (int kk = 0; kk < SUB; kk++)
@@ -237,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
         if (j < w) {
           const int64_t slot = base + (int64_t)j * C;
           v[u] = __builtin_nontemporal_load(a.val + slot);
-          cj[u] = __builtin_nontemporal_load(a.col + slot);
+          cj[u] = load_col(slot);
         } else {
           v[u] = T(0);
           cj[u] = 0;
@@ -275,7 +339,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
           if (j < w) {
             const int64_t slot = base + (int64_t)j * C;
             v[u] = __builtin_nontemporal_load(a.val + slot);
-            cj[u] = __builtin_nontemporal_load(a.col + slot);
+            cj[u] = load_col(slot);
           } else {
             v[u] = T(0);
             cj[u] = 0;
@@ -292,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
             const int32_t cc = group_bcast<LPR>(cj[k / LPR], k % LPR, lane);
             vv[kk] = group_bcast_t<LPR>(v[k / LPR], k % LPR, lane);
             ok[kk] = colok && (j0 + k) < w;
-            if (ok[kk]) vload<T, VPL>(xb + (int64_t)cc * a.ldx, xg[kk]);
+            if (ok[kk]) gather(cc, xg[kk]);
           }
 #pragma unroll
           for (int kk = 0; kk < SUB; kk++)
@@ -446,18 +510,19 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SpmvArgs<T> a) {
 
 namespace {
 
-template <typename T, int VPL, int LPR, bool XSC, int GR = 0, int SL = 8>
+template <typename T, int VPL, int LPR, bool XSC, int GR = 0, int SL = 8, bool STG = false>
 int launch_one(SpmvArgs<T>& a) {
-  auto kern = k_sell_spmv<T, VPL, LPR, XSC, GR, SL>;
+  auto kern = k_sell_spmv<T, VPL, LPR, XSC, GR, SL, STG>;
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
   a.ntiles = (a.nslots + RPB - 1) / RPB;
-  const int grid = persistent_blocks(kern, 0, a.ntiles, a.blocks_per_cu);
+  const size_t lds = STG ? a.st_lds_bytes : 0;
+  const int grid = persistent_blocks(kern, lds, a.ntiles, a.blocks_per_cu);
   if (GR > 0)
     a.partials = (T*)device_work(sizeof(T) * (size_t)2 * grid * a.ncols * a.ncols);
   else if (a.flags & (F_DWW | F_DVW))
     a.partials = (T*)device_work(sizeof(T) * ((size_t)grid * 2 * a.ncols + 2 * a.ncols));
   ktimer_begin();
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, rt().stream, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, rt().stream, a);
   PHX_HIP(hipGetLastError());
   ktimer_end();
   return grid;
@@ -500,6 +565,23 @@ int launch_xsc(SpmvArgs<T>& a, int lpr, int depth) {
   return a.xscale ? launch_lpr<T, VPL, true>(a, lpr) : launch_lpr<T, VPL, false>(a, lpr);
 }
 
+// staged windows (row groups of at most 8 lanes of 16-byte vectors): depth 8, or 16 for lanes of at most 32 bytes
+template <typename T, int VPL, int SL>
+int launch_stage_sl(SpmvArgs<T>& a, int lpr) {
+  switch (lpr) {
+    case 1: return launch_one<T, VPL, 1, false, 0, SL, true>(a);
+    case 2: return launch_one<T, VPL, 2, false, 0, SL, true>(a);
+    case 4: return launch_one<T, VPL, 4, false, 0, SL, true>(a);
+    default: return launch_one<T, VPL, 8, false, 0, SL, true>(a);
+  }
+}
+template <typename T, int VPL>
+int launch_stage(SpmvArgs<T>& a, int lpr, int depth) {
+  if constexpr (VPL * sizeof(T) <= 32)
+    if (depth >= 16) return launch_stage_sl<T, VPL, 16>(a, lpr);
+  return launch_stage_sl<T, VPL, 8>(a, lpr);
+}
+
 // the Gram-epilogue variants: at most 32 columns (lane groups of at most 32 lanes)
 template <typename T, int VPL, int GR>
 int launch_gram_lpr(SpmvArgs<T>& a, int lpr) {
@@ -529,6 +611,29 @@ __global__ void __launch_bounds__(kBlock) k_spmv_gram_out(const T* __restrict__ 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
+
+template <typename T>
+const SellStage* spmv_stage_for(const SparseMat<T>* A, int nc, int vpl, int* pshift, size_t* lds_bytes) {
+  if (vpl * (int)sizeof(T) < 16 || (nc * (int)sizeof(T)) % 16 != 0) return nullptr;
+  const int groups = (nc + vpl - 1) / vpl;
+  int lpr = 1;
+  while (lpr < groups) lpr <<= 1;
+  if (lpr > 8) return nullptr;
+  const SellStage* S = sell_stage(A, (kBlock / 64) * (64 / lpr));
+  if (!S) return nullptr;
+  const int ppr = nc * (int)sizeof(T) / 16;
+  int ps = 0;
+  while ((1 << ps) < ppr) ps++;
+  const size_t bytes = (size_t)S->max_rows * ((size_t)16 << ps);
+  // the kernel's own LDS: the dot partials of its four waves (2 x 4 x 64 lane groups' columns), and a margin
+  const int64_t static_lds = (int64_t)2 * (kBlock / 64) * 64 * vpl * (int64_t)sizeof(T) + 256;
+  if (bytes > (size_t)kStageLdsBytes || (int64_t)bytes + static_lds > rt().lds_per_block) return nullptr;
+  if (pshift) *pshift = ps;
+  if (lds_bytes) *lds_bytes = std::max<size_t>(16, bytes);
+  return S;
+}
+template const SellStage* spmv_stage_for<double>(const SparseMat<double>*, int, int, int*, size_t*);
+template const SellStage* spmv_stage_for<float>(const SparseMat<float>*, int, int, int*, size_t*);
 
 template <typename T>
 void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mvec<T>* x, T beta, Mvec<T>* y,
@@ -627,6 +732,15 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
   else
     depth = 8;
   const bool dots = req.dot_ww || req.dot_vw;
+  // x rows of each tile staged in LDS (where the matrix's tiles allow it)
+  int stage = req.stage;
+  if (stage < 0) stage = rt().spmv_stage;
+  if (stage < 0) stage = tuned != A->tuned.end() ? tuned->second.stage : 0;
+  a.st_ivptr = nullptr;
+  a.st_iv = nullptr;
+  a.st_lcol = nullptr;
+  a.st_ppr = a.st_pshift = 0;
+  a.st_lds_bytes = 0;
 
   for (int cb = 0; cb < nvec; cb += maxcols) {
     const int nc = std::min(maxcols, nvec - cb);
@@ -658,10 +772,22 @@ void spmv_launch(const SparseMat<T>* A, T alpha, const T* shifts_host, const Mve
       }
       continue;
     }
-    const int grid = VPL == 1     ? launch_xsc<T, 1>(a, lpr, depth)
-                     : VPL == V16 ? launch_xsc<T, V16>(a, lpr, depth)
-                     : VPL == 2 * V16 ? launch_xsc<T, 2 * V16>(a, lpr, depth)
-                                      : launch_xsc<T, 4 * V16>(a, lpr, depth);
+    const SellStage* S = stage > 0 && !a.xscale ? spmv_stage_for(A, nc, VPL, &a.st_pshift, &a.st_lds_bytes) : nullptr;
+    int grid;
+    if (S) {
+      rt().spmv_staged_launches++;
+      a.st_ivptr = S->d_ivptr;
+      a.st_iv = S->d_iv;
+      a.st_lcol = S->d_lcol;
+      a.st_ppr = nc * (int)sizeof(T) / 16;
+      grid = VPL == V16 ? launch_stage<T, V16>(a, lpr, depth)
+                        : (VPL == 2 * V16 ? launch_stage<T, 2 * V16>(a, lpr, depth) : launch_stage<T, 4 * V16>(a, lpr, depth));
+    } else {
+      grid = VPL == 1     ? launch_xsc<T, 1>(a, lpr, depth)
+             : VPL == V16 ? launch_xsc<T, V16>(a, lpr, depth)
+             : VPL == 2 * V16 ? launch_xsc<T, 2 * V16>(a, lpr, depth)
+                              : launch_xsc<T, 4 * V16>(a, lpr, depth);
+    }
     if (dots && req.dots_dev && cb == 0 && nc == nvec) {
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((2 * nc + 15) / 16), dim3(kBlock), 0, rt().stream, a.partials,
                          grid, 2 * nc, static_cast<T*>(req.dots_dev));

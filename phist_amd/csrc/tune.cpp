@@ -15,7 +15,9 @@
 //     and 32 also prefetch the next batch's indices);
 //   * per block width above 16 columns, launches of 16 (and above 32, of 32)
 //     columns instead of one: each reads the matrix again, but gathers rows
-//     of x half (a quarter) as wide, so fewer of them leave L2.
+//     of x half (a quarter) as wide, so fewer of them leave L2;
+//   * per block width, where every workgroup tile's x window fits LDS, the
+//     product that stages each tile's window in LDS and gathers from there.
 // The choice is stored on the matrix (SparseMat::tuned, used by every later
 // SpMV of that width unless a global setting overrides it) and cached for
 // the process by a matrix signature (scalar size, rows, entries, longest row,
@@ -102,13 +104,14 @@ struct TrialBlock {
 // take the library's usual choice (global settings, tuned entry, table)
 template <typename T>
 double time_product(const SparseMat<T>* A, Mvec<T>* x, Mvec<T>* y, int lb, int bpc, int depth, int reps,
-                    int col_block = -1) {
+                    int col_block = -1, int stage = -1) {
   Runtime& r = rt();
   SpmvRequest rq;
   rq.lane_bytes = lb;
   rq.blocks_per_cu = bpc;
   rq.depth = depth;
   rq.col_block = col_block;
+  rq.stage = stage;
   spmv_launch<T>(A, T(1), nullptr, x, T(0), y, rq, nullptr, nullptr);
   double best = 1e300;
   for (int k = 0; k < reps; k++) {
@@ -156,19 +159,25 @@ SpmvTuned search_launch(const SparseMat<T>* A, int nvec, int reps) {
       if (std::find(seen.begin(), seen.end(), vpl) != seen.end()) continue;
       seen.push_back(vpl);
       if (cbw > 0 && cbw % vpl != 0) continue;
-      std::vector<int> tried;
-      for (int d : depths) {
-        const int depth = spmv_effective_depth(d, vpl, (int)sizeof(T), ncol);
-        if (std::find(tried.begin(), tried.end(), depth) != tried.end()) continue;
-        tried.push_back(depth);
-        for (int bpc : {1, 2, 3, 4, 6, 8}) {
-          const double ms = time_product<T>(A, &x.v, &y.v, lb, bpc, depth, reps, cbw);
-          if (ms < best.ms) {
-            best.ms = ms;
-            best.lane_bytes = lb;
-            best.blocks_per_cu = bpc;
-            best.depth = depth;
-            best.col_block = cbw;
+      for (int stg : {0, 1}) {
+        // staged windows where every tile's window fits LDS (depth 8, or 16 for lanes of at most 32 bytes)
+        if (stg && !spmv_stage_for(A, ncol, vpl, nullptr, nullptr)) continue;
+        std::vector<int> tried;
+        for (int d : depths) {
+          const int depth = stg ? (d >= 16 && vpl * (int)sizeof(T) <= 32 ? 16 : 8)
+                                : spmv_effective_depth(d, vpl, (int)sizeof(T), ncol);
+          if (std::find(tried.begin(), tried.end(), depth) != tried.end()) continue;
+          tried.push_back(depth);
+          for (int bpc : {1, 2, 3, 4, 6, 8}) {
+            const double ms = time_product<T>(A, &x.v, &y.v, lb, bpc, depth, reps, cbw, stg);
+            if (ms < best.ms) {
+              best.ms = ms;
+              best.lane_bytes = lb;
+              best.blocks_per_cu = bpc;
+              best.depth = depth;
+              best.col_block = cbw;
+              best.stage = stg;
+            }
           }
         }
       }
@@ -179,7 +188,7 @@ SpmvTuned search_launch(const SparseMat<T>* A, int nvec, int reps) {
 
 }  // namespace
 
-// settings[6*i .. 6*i+5] = C, sigma, lane bytes, workgroups per CU, depth, column block for nvecs[i]
+// settings[7*i .. 7*i+6] = C, sigma, lane bytes, workgroups per CU, depth, column block, staging for nvecs[i]
 template <typename T>
 void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, double* default_ms, double* tuned_ms,
                  int* settings) {
@@ -255,12 +264,13 @@ void sparse_tune(SparseMat<T>* A, int n_nvec, const int* nvecs, int flags, doubl
     A->tuned[nvecs[i]].ms = ms;
     if (tuned_ms) tuned_ms[i] = ms;
     if (settings) {
-      settings[6 * i + 0] = A->C;
-      settings[6 * i + 1] = A->sigma;
-      settings[6 * i + 2] = chosen[i].launch.lane_bytes;
-      settings[6 * i + 3] = chosen[i].launch.blocks_per_cu;
-      settings[6 * i + 4] = chosen[i].launch.depth;
-      settings[6 * i + 5] = chosen[i].launch.col_block;
+      settings[7 * i + 0] = A->C;
+      settings[7 * i + 1] = A->sigma;
+      settings[7 * i + 2] = chosen[i].launch.lane_bytes;
+      settings[7 * i + 3] = chosen[i].launch.blocks_per_cu;
+      settings[7 * i + 4] = chosen[i].launch.depth;
+      settings[7 * i + 5] = chosen[i].launch.col_block;
+      settings[7 * i + 6] = chosen[i].launch.stage;
     }
   }
   sync_stream();

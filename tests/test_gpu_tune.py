@@ -21,13 +21,13 @@ def _tune(be, A, nvecs, flags):
     n = len(nvecs)
     nv = np.array(nvecs, dtype=np.int32)
     dms, tms = np.zeros(n), np.zeros(n)
-    st = np.zeros(6 * n, dtype=np.int32)
+    st = np.zeros(7 * n, dtype=np.int32)
     fl = ctypes.c_int(0)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     getattr(be.L, f"phist_{be.p}amd_sparseMat_tune")(A.h, ctypes.c_int(n), p(nv), ctypes.c_int(flags), p(dms),
                                                       p(tms), p(st), ctypes.byref(fl))
     assert fl.value == 0
-    return dms, tms, st.reshape(n, 6)
+    return dms, tms, st.reshape(n, 7)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -49,7 +49,7 @@ def test_tune_layout_and_launch(be64, be32, dtype):
         y.delete()
     dms, tms, st = _tune(be, A, [1, 8], 1 | 2)
     item = np.dtype(dtype).itemsize
-    for (C, sigma, lb, bpc, depth, cblk), d, t in zip(st, dms, tms):
+    for (C, sigma, lb, bpc, depth, cblk, stg), d, t in zip(st, dms, tms):
         assert C in (16, 32, 64) and sigma in (1, 256)
         assert lb in (0, item, 32, 64) and bpc in (1, 2, 3, 4, 6, 8) and depth in (8, 16, 32)
         assert cblk == 0  # widths 1 and 8: one launch
@@ -205,3 +205,121 @@ def test_spmv_column_blocks_bitexact(be64, be32, dtype, nvec):
         o.delete()
     m.delete()
 
+
+
+def _staged(be):
+    c, fl = ctypes.c_int64(0), ctypes.c_int(0)
+    be.L.phist_amd_spmv_staged_launches(ctypes.byref(c), ctypes.byref(fl))
+    assert fl.value == 0
+    return c.value
+
+
+def _stage_expected(nvec, item, lane_bytes):
+    """whether a one-launch SpMV of nvec contiguous columns can stage x (the library's rule:
+    lanes of whole 16-byte vectors, at most 8 lanes per row), the LDS window fitting"""
+    v16 = 16 // item
+    vpl = v16
+    if lane_bytes == 32 and nvec % (32 // item) == 0:
+        vpl = 32 // item
+    if nvec % v16:
+        return False
+    groups, lpr = -(-nvec // vpl), 1
+    while lpr < groups:
+        lpr *= 2
+    return lpr <= 8
+
+
+@pytest.mark.parametrize("depth", [8, 16])
+@pytest.mark.parametrize("lane_bytes", [0, 32])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nvec", [2, 4, 8, 16, 32])
+def test_spmv_stage_bitexact(be64, be32, dtype, nvec, lane_bytes, depth):
+    """x staged in LDS per workgroup tile (phist_amd_set_spmv_stage) gives the unstaged product bit for
+    bit, shifts and beta included, and the fused column dots to rounding, on a 3D Laplacian (tiles
+    gather from three short intervals: staged wherever the block rows are whole 16-byte vectors in
+    at most 8 lanes), a narrow band (one interval per tile) and a honeycomb sheet; a wide band whose
+    windows do not fit falls back to the unstaged kernel; all agree with the oracle"""
+    be = be64 if dtype == np.float64 else be32
+    item = np.dtype(dtype).itemsize
+    fl = ctypes.c_int(0)
+    cases = [("laplace3d", lambda: be.laplace3d(20, 21, 22), True),
+             ("band300", lambda: be.banded_random(30000, 20, 300, 40.0, 5), None),
+             ("honeycomb", lambda: be.honeycomb(60, 80, 0, 0, 0, 1.0, 7), None),
+             ("band20k", lambda: be.banded_random(60000, 24, 20000, 40.0, 6), False)]
+    for name, make, expect in cases:
+        A = make()
+        n = A.nrows
+        ro, ci, va = A.crs()
+        ref = ol.Crs(n, n, ro, ci, va.astype(float))
+        rng = np.random.default_rng(nvec + 100 * depth)
+        X = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+        Y0 = rng.uniform(-1, 1, (n, nvec)).astype(dtype)
+        shifts = rng.uniform(-2, 2, nvec).astype(dtype)
+        m = be.map(n)
+        x = be.mvec(m, nvec, X)
+        out = {}
+        try:
+            be.L.phist_amd_set_spmv_lane_bytes(lane_bytes, ctypes.byref(fl))
+            be.L.phist_amd_set_spmv_depth(depth, ctypes.byref(fl))
+            for stage in (0, 1):
+                be.L.phist_amd_set_spmv_stage(stage, ctypes.byref(fl))
+                assert fl.value == 0
+                c0 = _staged(be)
+                y = be.mvec(m, nvec, Y0)
+                be.spmv_shifted(0.7, A, shifts, x, 0.3, y)
+                w = be.mvec(m, nvec, Y0)
+                ww, vw = be.fused_spmv_mvdot(1.1, A, x, 0.0, w)
+                out[stage] = (y.get(), w.get(), ww, vw, _staged(be) - c0)
+                y.delete()
+                w.delete()
+        finally:
+            be.L.phist_amd_set_spmv_stage(-1, ctypes.byref(fl))
+            be.L.phist_amd_set_spmv_depth(0, ctypes.byref(fl))
+            be.L.phist_amd_set_spmv_lane_bytes(0, ctypes.byref(fl))
+        assert out[0][4] == 0
+        if expect is True:
+            assert (out[1][4] == 2) == _stage_expected(nvec, item, lane_bytes), (name, out[1][4])
+        elif expect is False:
+            assert out[1][4] == 0, name
+        assert np.array_equal(out[1][0], out[0][0]), f"{name}: staged shifted product differs"
+        assert np.array_equal(out[1][1], out[0][1]), f"{name}: staged fused product differs"
+        for g, d in zip(out[1][2:4], out[0][2:4]):
+            assert np.max(np.abs(g - d) / np.abs(d)) <= (1e-13 if dtype == np.float64 else 1e-5)
+        want = ol.spmv(ref, 0.7, X.astype(float), 0.3, Y0.astype(float), shifts=shifts.astype(float))
+        num = np.linalg.norm(out[1][0].astype(float) - want, axis=0)
+        assert np.all(num <= (1e-12 if dtype == np.float64 else 1e-5) * np.linalg.norm(want, axis=0))
+        x.delete()
+        A.delete()
+        m.delete()
+
+
+def test_spmv_stage_tuned_laplacian(be64):
+    """the tuner weighs staging on a 3D Laplacian at 4 and 8 columns (reported as the 7th setting); the
+    product under its choice equals the unstaged one bit for bit"""
+    n = 32 ** 3
+    A = be64.laplace3d(32, 32, 32)
+    m = be64.map(n)
+    rng = np.random.default_rng(11)
+    res = {}
+    for nvec in (4, 8):
+        X = rng.uniform(-1, 1, (n, nvec))
+        x, y = be64.mvec(m, nvec, X), be64.mvec(m, nvec)
+        fl = ctypes.c_int(0)
+        be64.L.phist_amd_set_spmv_stage(0, ctypes.byref(fl))
+        try:
+            be64.spmv(1.0, A, x, 0.0, y)
+        finally:
+            be64.L.phist_amd_set_spmv_stage(-1, ctypes.byref(fl))
+        res[nvec] = (X, y.get())
+        x.delete()
+        y.delete()
+    _, _, st = _tune(be64, A, [4, 8], 2)
+    assert set(st[:, 6].tolist()) <= {0, 1}
+    for nvec, (X, Y0) in res.items():
+        x, y = be64.mvec(m, nvec, X), be64.mvec(m, nvec)
+        be64.spmv(1.0, A, x, 0.0, y)
+        assert np.array_equal(y.get(), Y0), nvec
+        x.delete()
+        y.delete()
+    A.delete()
+    m.delete()
