@@ -83,6 +83,14 @@ struct Runtime {
   // pinned host scratch for reduction results
   void* h_work = nullptr;
   size_t h_work_bytes = 0;
+  // results of host reductions published by the device: coherent pinned
+  // memory, a sequence word at offset 0 and the values from kSignalData on
+  // (signal_buffer); fast_sync = 0 takes the copy + stream synchronisation
+  static constexpr size_t kSignalBytes = 64 * 1024, kSignalData = 256;
+  void* h_sig = nullptr;
+  void* d_sig = nullptr;
+  uint32_t sig_seq = 0;
+  int fast_sync = 1;
   // pinned staging slots of stage_upload_2d, each with the event of its last copy
   static constexpr int kStageSlots = 64;
   static constexpr size_t kStageSlotBytes = 64 * 1024;
@@ -175,6 +183,13 @@ Runtime& rt();
 void ensure_init();
 void* device_work(size_t bytes);
 void* host_work(size_t bytes);
+// the published-result buffer (host and device addresses of the same coherent
+// pinned memory), allocated on first use; false where fast_sync is off or
+// host reductions are refused (allocation-free calls, graph capture)
+bool signal_buffer(void** host, void** dev);
+// waits until the device has written seq to the buffer's sequence word: a
+// short spin, then a stream synchronisation (after which the word must hold seq)
+void wait_signal(uint32_t seq);
 void sync_stream();
 // Fixed device scratch of the allocation-free entry points: kNoAllocScratchBytes
 // per device, held until the process ends.  noalloc_scratch() returns this

@@ -156,11 +156,46 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(const T* __restrict_
   if (threadIdx.x < 16 && o < width) out[o] = s;
 }
 
+// The sums of k_reduce_partials written to the published-result buffer (host
+// memory): with one workgroup the sums go there directly, else they are read
+// from out; then the buffer's sequence word becomes seq.  The sums and the
+// word are written by the first wave, whose system-scope release store of
+// the word follows its stores of the sums.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_reduce_publish(const T* __restrict__ partials, int nblocks, int width,
+                                                           const T* __restrict__ out, T* pub, uint32_t* word,
+                                                           uint32_t seq) {
+  if (width <= 16) {
+    const T s = block_sum16(partials, nblocks, width);
+    if (threadIdx.x < 16 && (int)threadIdx.x < width) pub[threadIdx.x] = s;
+  } else if (threadIdx.x < 64) {
+    for (int o = threadIdx.x; o < width; o += 64) pub[o] = out[o];
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(word, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // reduce [nblocks][width] device partials and copy the width sums to host_out
 template <typename T>
 inline void reduce_partials_to_host(const T* d_partials, int nblocks, int width, T* host_out) {
   Runtime& r = rt();
   T* d_out = const_cast<T*>(d_partials) + (size_t)nblocks * width;
+  void *hs = nullptr, *ds = nullptr;
+  if ((size_t)width * sizeof(T) <= Runtime::kSignalBytes - Runtime::kSignalData && signal_buffer(&hs, &ds)) {
+    // the device publishes the sums and a new sequence number; the host waits for that number
+    if (width > 16) {
+      hipLaunchKernelGGL(k_reduce_partials<T>, dim3((width + 15) / 16), dim3(kBlock), 0, r.stream, d_partials,
+                         nblocks, width, d_out);
+      PHX_HIP(hipGetLastError());
+    }
+    const uint32_t seq = ++r.sig_seq;
+    hipLaunchKernelGGL(k_reduce_publish<T>, dim3(1), dim3(kBlock), 0, r.stream, d_partials, nblocks, width, d_out,
+                       reinterpret_cast<T*>(static_cast<char*>(ds) + Runtime::kSignalData), static_cast<uint32_t*>(ds),
+                       seq);
+    PHX_HIP(hipGetLastError());
+    wait_signal(seq);
+    memcpy(host_out, static_cast<const char*>(hs) + Runtime::kSignalData, sizeof(T) * width);
+    return;
+  }
   hipLaunchKernelGGL(k_reduce_partials<T>, dim3((width + 15) / 16), dim3(kBlock), 0, r.stream, d_partials, nblocks,
                      width, d_out);
   PHX_HIP(hipGetLastError());

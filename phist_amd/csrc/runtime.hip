@@ -6,6 +6,7 @@
 // kernels_common_impl.cpp:23-88), src/kernels/builtin/map_module.F90:41-66,
 // 289-320 (contiguous distribution; ilower/iupper are 0-based).
 #include "amd_common.hpp"
+#include <chrono>
 #include <cstdarg>
 #include <map>
 #include <tuple>
@@ -90,6 +91,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_SPMV_DEPTH")) r.spmv_depth = atoi(s);
   if (const char* s = getenv("PHIST_AMD_SPMV_COL_BLOCK")) r.spmv_col_block = std::max(-1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_SPMV_STAGE")) r.spmv_stage = std::max(-1, std::min(1, atoi(s)));
+  if (const char* s = getenv("PHIST_AMD_FAST_SYNC")) r.fast_sync = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_SPMV_NT_Y")) r.spmv_nt_y = atoi(s) < 0 ? -1 : (atoi(s) != 0 ? 1 : 0);
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA")) r.gram_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_MFMA_WIDE")) r.gram_mfma_wide = std::max(1, atoi(s));
@@ -150,6 +152,43 @@ void* host_work(size_t bytes) {
     r.h_work_bytes = nb;
   }
   return r.h_work;
+}
+
+bool signal_buffer(void** host, void** dev) {
+  Runtime& r = rt();
+  if (!r.fast_sync || r.fixed_work || r.capturing) return false;
+  if (!r.h_sig) {
+    void* h = nullptr;
+    PHX_HIP(hipHostMalloc(&h, Runtime::kSignalBytes, hipHostMallocCoherent | hipHostMallocMapped));
+    void* d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess || !d) {
+      PHX_HIP(hipHostFree(h));
+      r.fast_sync = 0;
+      return false;
+    }
+    memset(h, 0, Runtime::kSignalBytes);
+    r.h_sig = h;
+    r.d_sig = d;
+  }
+  *host = r.h_sig;
+  *dev = r.d_sig;
+  return true;
+}
+
+void wait_signal(uint32_t seq) {
+  Runtime& r = rt();
+  const uint32_t* word = static_cast<const uint32_t*>(r.h_sig);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int spin = 1;; spin++) {
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+    if ((spin & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) break;
+    __builtin_ia32_pause();
+  }
+  // long waits (much queued work) block in the runtime instead of spinning
+  PHX_HIP(hipStreamSynchronize(r.stream));
+  PHX_REQUIRE(__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq, PHIST_FUNCTIONAL_ERROR,
+              "published reduction %u not seen after the stream finished", seq);
 }
 
 void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
