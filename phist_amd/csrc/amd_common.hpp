@@ -190,6 +190,11 @@ bool signal_buffer(void** host, void** dev);
 // waits until the device has written seq to the buffer's sequence word: a
 // short spin, then a stream synchronisation (after which the word must hold seq)
 void wait_signal(uint32_t seq);
+// copies bytes from device memory to host memory through the published-result
+// buffer (a one-wave kernel writes them and a new sequence number; the host
+// waits for the number); false, with nothing copied, where that buffer is
+// unavailable or too small or the source is not 4-byte aligned
+bool fetch_to_host(void* host, const void* dev, size_t bytes);
 void sync_stream();
 // Fixed device scratch of the allocation-free entry points: kNoAllocScratchBytes
 // per device, held until the process ends.  noalloc_scratch() returns this

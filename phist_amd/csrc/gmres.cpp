@@ -212,8 +212,10 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
         gmres_cgs_dev<TI>(1, nrows, v, ldv, w->dev(), w->lda, kk, nl, D1, d_act, dbuf, work_elems, D2);
         gmres_cgs_dev<TI>(2, nrows, v, ldv, w->dev(), w->lda, kk, nl, D2, d_act, dbuf, work_elems, N);
         TI* hb = static_cast<TI*>(host_work(sizeof(TI) * (2 * nc + kk)));
-        PHX_HIP(hipMemcpyAsync(hb, d_outs, sizeof(TI) * (2 * nc + kk), hipMemcpyDeviceToHost, rt().stream));
-        PHX_HIP(hipStreamSynchronize(rt().stream));
+        if (!fetch_to_host(hb, d_outs, sizeof(TI) * (2 * nc + kk))) {
+          PHX_HIP(hipMemcpyAsync(hb, d_outs, sizeof(TI) * (2 * nc + kk), hipMemcpyDeviceToHost, rt().stream));
+          PHX_HIP(hipStreamSynchronize(rt().stream));
+        }
         for (int a = 0; a < kk; a++) {
           if (active[a])
             for (int l = 0; l <= i; l++) hc[(size_t)l * kk + a] = (double)hb[l * kk + a] + (double)hb[nc + l * kk + a];

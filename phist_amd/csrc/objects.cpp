@@ -144,12 +144,16 @@ void copy_rows_d2h(void* host, size_t hp, const void* dev, size_t dp, size_t wid
     src = stage;
   }
   if (hp == width) {
-    PHX_HIP(hipMemcpyAsync(host, src, width * rows, hipMemcpyDeviceToHost, st));
-    sync_stream();
+    if (!fetch_to_host(host, src, width * rows)) {
+      PHX_HIP(hipMemcpyAsync(host, src, width * rows, hipMemcpyDeviceToHost, st));
+      sync_stream();
+    }
   } else {
     std::vector<char> pack(width * rows);
-    PHX_HIP(hipMemcpyAsync(pack.data(), src, width * rows, hipMemcpyDeviceToHost, st));
-    sync_stream();
+    if (!fetch_to_host(pack.data(), src, width * rows)) {
+      PHX_HIP(hipMemcpyAsync(pack.data(), src, width * rows, hipMemcpyDeviceToHost, st));
+      sync_stream();
+    }
     char* dst = static_cast<char*>(host);
     for (size_t r = 0; r < rows; r++) memcpy(dst + r * hp, pack.data() + r * width, width);
   }
@@ -340,6 +344,20 @@ void sd_to_device(const SdMat<T>* M) {
 
 template <typename T>
 void sd_from_device(const SdMat<T>* M) {
+  if (M->nrows == 0 || M->ncols == 0) return;
+  // the column span [first entry, last entry] through the published-result buffer when it fits
+  const int ld = M->root->ld;
+  const size_t span = (size_t)ld * (M->ncols - 1) + M->nrows;
+  if (M->nrows == ld) {
+    if (fetch_to_host(M->host(), M->dev(), sizeof(T) * span)) return;
+  } else if (sizeof(T) * span <= Runtime::kSignalBytes - Runtime::kSignalData) {
+    std::vector<T> tmp(span);
+    if (fetch_to_host(tmp.data(), M->dev(), sizeof(T) * span)) {
+      for (int c = 0; c < M->ncols; c++)
+        memcpy(M->host() + (size_t)c * ld, tmp.data() + (size_t)c * ld, sizeof(T) * M->nrows);
+      return;
+    }
+  }
   sd_from_device_async(M);
   sync_stream();
 }

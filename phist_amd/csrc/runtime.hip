@@ -191,6 +191,52 @@ void wait_signal(uint32_t seq) {
               "published reduction %u not seen after the stream finished", seq);
 }
 
+namespace {
+// one wave copies nwords words to the published-result buffer, eight per lane
+// in flight, then stores the sequence number (system-scope release)
+template <typename W>
+__global__ void __launch_bounds__(64) k_publish_words(const W* __restrict__ src, int64_t nwords, W* pub, uint32_t* word,
+                                                      uint32_t seq) {
+  constexpr int U = 8;
+  for (int64_t b = 0; b < nwords; b += 64 * U) {
+    W v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = b + u * 64 + threadIdx.x;
+      if (i < nwords) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = b + u * 64 + threadIdx.x;
+      if (i < nwords) pub[i] = v[u];
+    }
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(word, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+}  // namespace
+
+bool fetch_to_host(void* host, const void* dev, size_t bytes) {
+  void *hs = nullptr, *ds = nullptr;
+  if (bytes == 0 || bytes > Runtime::kSignalBytes - Runtime::kSignalData) return false;
+  const uintptr_t al = reinterpret_cast<uintptr_t>(dev) | bytes;
+  if (al % 4 != 0 || !signal_buffer(&hs, &ds)) return false;
+  Runtime& r = rt();
+  const uint32_t seq = ++r.sig_seq;
+  char* pub = static_cast<char*>(ds) + Runtime::kSignalData;
+  uint32_t* word = static_cast<uint32_t*>(ds);
+  typedef unsigned int w4_t __attribute__((ext_vector_type(4)));
+  if (al % 16 == 0)
+    hipLaunchKernelGGL(k_publish_words<w4_t>, dim3(1), dim3(64), 0, r.stream, static_cast<const w4_t*>(dev),
+                       (int64_t)(bytes / 16), reinterpret_cast<w4_t*>(pub), word, seq);
+  else
+    hipLaunchKernelGGL(k_publish_words<uint32_t>, dim3(1), dim3(64), 0, r.stream, static_cast<const uint32_t*>(dev),
+                       (int64_t)(bytes / 4), reinterpret_cast<uint32_t*>(pub), word, seq);
+  PHX_HIP(hipGetLastError());
+  wait_signal(seq);
+  memcpy(host, static_cast<const char*>(hs) + Runtime::kSignalData, bytes);
+  return true;
+}
+
 void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
 
 // The occupancy query costs microseconds of host time per call, as much as a
