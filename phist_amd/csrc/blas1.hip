@@ -11,7 +11,9 @@
 //
 // Same mapping as the SpMV: a row is owned by LPR lanes holding VPL
 // consecutive columns each, workgroups walk row tiles on a persistent
-// XCD-aware grid, reductions are fixed-order two-stage.
+// XCD-aware grid, reductions are fixed-order two-stage.  The streaming
+// kernels take kRowUnroll rows per lane group and tile, whose loads are all
+// in flight before the first store.
 #include "amd_device.hpp"
 #include <algorithm>
 
@@ -35,6 +37,9 @@ struct RowArgs {
 
 enum : int { B_READX = 1, B_READY = 2 };
 
+// rows per lane group and tile of the streaming kernels (k_axpby, k_elemwise)
+constexpr int kRowUnroll = 4;
+
 template <typename T, int VPL, int LPR>
 __global__ void __launch_bounds__(kBlock) k_axpby(const RowArgs<T> p) {
   constexpr int RPW = 64 / LPR, RPB = (kBlock / 64) * RPW;
@@ -47,27 +52,35 @@ __global__ void __launch_bounds__(kBlock) k_axpby(const RowArgs<T> p) {
     ca[e] = p.a[c0 + e];
     cb[e] = p.b[c0 + e];
   }
+  // a tile is kRowUnroll sub-tiles of RPB rows: all their loads go out before the first store
   const TileWalk tw = xcd_tiles(p.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
-    const int64_t i = t * RPB + wv * RPW + rw;
-    if (i >= p.nrows) continue;
-    T out[VPL];
+    int64_t iu[kRowUnroll];
+    T xv[kRowUnroll][VPL], yv[kRowUnroll][VPL];
 #pragma unroll
-    for (int e = 0; e < VPL; e++) out[e] = T(0);
-    T* yp = p.y + i * p.ldy + c0;
-    if (p.flags & B_READX) {
-      T xv[VPL];
-      vload<T, VPL>(p.x + i * p.ldx + c0, xv);
-#pragma unroll
-      for (int e = 0; e < VPL; e++) out[e] = ca[e] * xv[e];
+    for (int u = 0; u < kRowUnroll; u++) {
+      iu[u] = (t * kRowUnroll + u) * RPB + wv * RPW + rw;
+      if (iu[u] < p.nrows) {
+        if (p.flags & B_READX) vload<T, VPL>(p.x + iu[u] * p.ldx + c0, xv[u]);
+        if (p.flags & B_READY) vload<T, VPL>(p.y + iu[u] * p.ldy + c0, yv[u]);
+      }
     }
-    if (p.flags & B_READY) {
-      T yv[VPL];
-      vload<T, VPL>(yp, yv);
 #pragma unroll
-      for (int e = 0; e < VPL; e++) out[e] = (p.flags & B_READX) ? out[e] + cb[e] * yv[e] : cb[e] * yv[e];
+    for (int u = 0; u < kRowUnroll; u++) {
+      if (iu[u] >= p.nrows) continue;
+      T out[VPL];
+#pragma unroll
+      for (int e = 0; e < VPL; e++) out[e] = T(0);
+      if (p.flags & B_READX) {
+#pragma unroll
+        for (int e = 0; e < VPL; e++) out[e] = ca[e] * xv[u][e];
+      }
+      if (p.flags & B_READY) {
+#pragma unroll
+        for (int e = 0; e < VPL; e++) out[e] = (p.flags & B_READX) ? out[e] + cb[e] * yv[u][e] : cb[e] * yv[u][e];
+      }
+      vstore<T, VPL>(p.y + iu[u] * p.ldy + c0, out);
     }
-    vstore<T, VPL>(yp, out);
   }
 }
 
@@ -79,15 +92,23 @@ __global__ void __launch_bounds__(kBlock) k_elemwise(const RowArgs<T> p) {
   if (c0 >= p.ncols) return;
   const TileWalk tw = xcd_tiles(p.ntiles);
   for (int64_t t = tw.t; t < tw.end; t += tw.stride) {
-    const int64_t i = t * RPB + wv * RPW + rw;
-    if (i >= p.nrows) continue;
-    T xv[VPL], yv[VPL];
-    vload<T, VPL>(p.x + i * p.ldx + c0, xv);
-    T* yp = p.y + i * p.ldy + c0;
-    vload<T, VPL>(yp, yv);
+    int64_t iu[kRowUnroll];
+    T xv[kRowUnroll][VPL], yv[kRowUnroll][VPL];
 #pragma unroll
-    for (int e = 0; e < VPL; e++) yv[e] = p.alpha * xv[e] * yv[e];
-    vstore<T, VPL>(yp, yv);
+    for (int u = 0; u < kRowUnroll; u++) {
+      iu[u] = (t * kRowUnroll + u) * RPB + wv * RPW + rw;
+      if (iu[u] < p.nrows) {
+        vload<T, VPL>(p.x + iu[u] * p.ldx + c0, xv[u]);
+        vload<T, VPL>(p.y + iu[u] * p.ldy + c0, yv[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRowUnroll; u++) {
+      if (iu[u] >= p.nrows) continue;
+#pragma unroll
+      for (int e = 0; e < VPL; e++) yv[u][e] = p.alpha * xv[u][e] * yv[u][e];
+      vstore<T, VPL>(p.y + iu[u] * p.ldy + c0, yv[u]);
+    }
   }
 }
 
@@ -135,7 +156,8 @@ enum Kind { K_AXPBY, K_ELEM, K_DOT };
 template <typename T, int VPL, int LPR>
 int launch_kind(Kind kind, RowArgs<T>& p) {
   constexpr int RPB = (kBlock / 64) * (64 / LPR);
-  p.ntiles = (p.nrows + RPB - 1) / RPB;
+  const int64_t rows_per_tile = kind == K_DOT ? RPB : (int64_t)RPB * kRowUnroll;
+  p.ntiles = (p.nrows + rows_per_tile - 1) / rows_per_tile;
   int grid;
   switch (kind) {
     case K_AXPBY:
