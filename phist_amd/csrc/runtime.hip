@@ -159,10 +159,15 @@ bool signal_buffer(void** host, void** dev) {
   if (!r.fast_sync || r.fixed_work || r.capturing) return false;
   if (!r.h_sig) {
     void* h = nullptr;
-    PHX_HIP(hipHostMalloc(&h, Runtime::kSignalBytes, hipHostMallocCoherent | hipHostMallocMapped));
+    if (hipHostMalloc(&h, Runtime::kSignalBytes, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess || !h) {
+      (void)hipGetLastError();  // not fatal: the copy + synchronisation path serves instead
+      r.fast_sync = 0;
+      return false;
+    }
     void* d = nullptr;
     const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
     if (e != hipSuccess || !d) {
+      (void)hipGetLastError();
       PHX_HIP(hipHostFree(h));
       r.fast_sync = 0;
       return false;
