@@ -60,6 +60,12 @@ void phist_amd_set_spmv_col_block(int ncols, int* iflag);
  * -1 = the default: the matrix's tuned value, else off.  Every choice gives
  * the same result bit for bit. */
 void phist_amd_set_spmv_stage(int mode, int* iflag);
+/* results of host reductions and small device-to-host copies (dots, norms,
+ * sdMat downloads): enable = 1 (the default, PHIST_AMD_FAST_SYNC) lets the
+ * device write them with a sequence number into coherent pinned memory and
+ * the host wait for that number; 0 copies them and synchronises the stream.
+ * The values are the same either way. */
+void phist_amd_set_fast_sync(int enable, int* iflag);
 /* SpMV launches so far (this process) that staged x in LDS */
 void phist_amd_spmv_staged_launches(int64_t* count, int* iflag);
 /* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16

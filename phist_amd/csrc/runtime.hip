@@ -864,6 +864,13 @@ void phist_amd_set_spmv_stage(int mode, int* iflag) {
     *iflag = 0;
   });
 }
+void phist_amd_set_fast_sync(int enable, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    rt().fast_sync = enable != 0;
+    *iflag = 0;
+  });
+}
 void phist_amd_spmv_staged_launches(int64_t* count, int* iflag) {
   guard(iflag, __func__, [&] {
     ensure_init();

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
+import phist_amd
 
 pytestmark = pytest.mark.gpu
 RTOL = {np.float64: 1e-12, np.float32: 1e-5}
@@ -505,3 +506,53 @@ def test_host_mirror_roundtrip(be64):
     be64.call("mvec_from_device", v.h)
     assert raw[5] == 2.0
     v.delete()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fast_sync_same_results(be64, be32, dtype):
+    """host reductions and small downloads give the same values whether the device publishes them
+    into pinned memory (phist_amd_set_fast_sync(1), the default) or they are copied after a stream
+    synchronisation (0): column dots and norms of 1..40 columns (one or several reduction
+    workgroups), the fused SpMV's dots, and sdMat downloads of whole and viewed blocks"""
+    be = be64 if dtype == np.float64 else be32
+    fl = ctypes.c_int(0)
+    n = 50000
+    m = be.map(n)
+    A = be.laplace3d(50, 40, 25)
+    rng = np.random.default_rng(21)
+    res = {}
+    try:
+        for fast in (0, 1):
+            be.L.phist_amd_set_fast_sync(fast, ctypes.byref(fl))
+            assert fl.value == 0
+            out = []
+            for k in (1, 4, 8, 20, 40):
+                X = rng.uniform(-1, 1, (n, k)).astype(dtype) if fast == 0 else res[0]["X"][k]
+                Y = rng.uniform(-1, 1, (n, k)).astype(dtype) if fast == 0 else res[0]["Y"][k]
+                x, y = be.mvec(m, k, X), be.mvec(m, k, Y)
+                out.append(be.dot(x, y))
+                out.append(be.norm2(x))
+                w = be.mvec(m, k)
+                out.extend(be.fused_spmv_mvdot(1.0, A, x, 0.0, w))
+                C = be.sdmat(k, k)
+                be.gram(1.0, x, y, 0.0, C)
+                out.append(C.get())
+                if k >= 4:
+                    Cv = ctypes.c_void_p()
+                    be.call("sdMat_view_block", C.h, ctypes.byref(Cv), 1, k - 2, 1, k - 1)
+                    V = phist_amd.SdMat(be, k - 2, k - 1, Cv)
+                    out.append(V.get())
+                    V.delete()
+                for o in (x, y, w, C):
+                    o.delete()
+                if fast == 0:
+                    res.setdefault(0, {"X": {}, "Y": {}})
+                    res[0]["X"][k], res[0]["Y"][k] = X, Y
+            res.setdefault(fast, {})["out"] = out
+    finally:
+        be.L.phist_amd_set_fast_sync(1, ctypes.byref(fl))
+    assert len(res[0]["out"]) == len(res[1]["out"])
+    for a, b in zip(res[0]["out"], res[1]["out"]):
+        assert np.array_equal(a, b)
+    A.delete()
+    m.delete()
