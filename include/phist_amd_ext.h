@@ -66,6 +66,12 @@ void phist_amd_set_spmv_stage(int mode, int* iflag);
  * the host wait for that number; 0 copies them and synchronises the stream.
  * The values are the same either way. */
 void phist_amd_set_fast_sync(int enable, int* iflag);
+/* blocked GMRES with the per-system Gram-Schmidt sweeps: enable = 1 (the
+ * default, PHIST_AMD_GMRES_DEVICE) runs the Givens rotations on the device
+ * and lets the host enqueue the next Arnoldi step before the previous one's
+ * convergence flags arrive; 0 computes them on the host after each step.
+ * The two agree to rounding (the rotations' hypot may differ in the last bit). */
+void phist_amd_set_gmres_device(int enable, int* iflag);
 /* SpMV launches so far (this process) that staged x in LDS */
 void phist_amd_spmv_staged_launches(int64_t* count, int* iflag);
 /* SpMV autotuner.  For each block width nvecs[0..n_nvec) (1..256, at most 16

@@ -145,6 +145,9 @@ struct Runtime {
   // blocked GMRES: Gram-Schmidt with the per-system sweeps (gmres_cgs_dev)
   // where the block allows, else with Grams and row mixes of the whole basis
   bool gmres_cgs = true;
+  // ... and with the sweeps, the Givens rotations on the device (the host
+  // runs one Arnoldi step ahead, reading the previous step's active flags)
+  bool gmres_device = true;
   // service loop: most MINRES iterations per batch and step while eigenvalue requests run
   int serve_max_reps = 4;  // (rate 40/s mix: p99 92.4 ms at 1, 84.3 at 4, 84.6 at 8)
   // device MINRES: iterations enqueued per iterate() call (at most; fewer when
@@ -195,6 +198,15 @@ void wait_signal(uint32_t seq);
 // waits for the number); false, with nothing copied, where that buffer is
 // unavailable or too small or the source is not 4-byte aligned
 bool fetch_to_host(void* host, const void* dev, size_t bytes);
+// asynchronous form for loops that run ahead of their results: enqueues the
+// copy of bytes (at most kSignalSlotBytes) into published slot `slot` (1 or
+// 2; fetch_to_host uses slot 0) and returns its sequence number (0: the
+// buffer is unavailable, nothing enqueued); published_data waits for that
+// number or a later one and returns the slot's host address.  A slot must be
+// read before it is written again.
+constexpr size_t kSignalSlotBytes = 4096;
+uint32_t publish_async(const void* dev, size_t bytes, int slot);
+const void* published_data(uint32_t seq, int slot);
 void sync_stream();
 // Fixed device scratch of the allocation-free entry points: kNoAllocScratchBytes
 // per device, held until the process ends.  noalloc_scratch() returns this
@@ -466,6 +478,16 @@ struct SpmvRequest {
 // limits of the blocked GMRES Gram-Schmidt sweeps (systems per block, basis
 // vectors per system held in registers)
 constexpr int kGmresCgsMaxKk = 64, kGmresCgsMaxNl = 32;
+
+// device state of the blocked GMRES's Givens recurrences for kk systems and
+// basis size m (double): H (per system (m+1) x m, column-major), cs, sn (m
+// per system), g (m+1), and per system ||r0||, tol, |g_{i+1}|/||r0||, steps
+// this cycle, status (1 iterating, 0 converged, 3 failed) and iterations
+// before this cycle
+struct GmresDevState {
+  double *H, *cs, *sn, *g, *normR0, *tol, *relres;
+  int *steps, *status, *iters0;
+};
 
 // per-column state of the device-resident MINRES recurrences (Paige & Saunders)
 struct MinresState {

@@ -180,14 +180,16 @@ inline void reduce_partials_to_host(const T* d_partials, int nblocks, int width,
   Runtime& r = rt();
   T* d_out = const_cast<T*>(d_partials) + (size_t)nblocks * width;
   void *hs = nullptr, *ds = nullptr;
-  if ((size_t)width * sizeof(T) <= Runtime::kSignalBytes - Runtime::kSignalData && signal_buffer(&hs, &ds)) {
+  if ((size_t)width * sizeof(T) <= Runtime::kSignalBytes - Runtime::kSignalData - 2 * kSignalSlotBytes &&
+      signal_buffer(&hs, &ds)) {
     // the device publishes the sums and a new sequence number; the host waits for that number
     if (width > 16) {
       hipLaunchKernelGGL(k_reduce_partials<T>, dim3((width + 15) / 16), dim3(kBlock), 0, r.stream, d_partials,
                          nblocks, width, d_out);
       PHX_HIP(hipGetLastError());
     }
-    const uint32_t seq = ++r.sig_seq;
+    uint32_t seq = ++r.sig_seq;
+    if (seq == 0) seq = ++r.sig_seq;
     hipLaunchKernelGGL(k_reduce_publish<T>, dim3(1), dim3(kBlock), 0, r.stream, d_partials, nblocks, width, d_out,
                        reinterpret_cast<T*>(static_cast<char*>(ds) + Runtime::kSignalData), static_cast<uint32_t*>(ds),
                        seq);

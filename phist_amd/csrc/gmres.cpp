@@ -17,7 +17,8 @@
 //   h(i+1) = ||w||, v_{i+1} = w / h(i+1)
 // (for more than 64 systems or m > 32: a Gram and a row mix of the whole
 // basis per pass, of which only the blocks' diagonals are used).
-//   Givens rotations of every system's Hessenberg column on the host; the
+//   Givens rotations of every system's Hessenberg column on the device (one
+//   thread per system; on the host for the whole-basis passes); the
 //   implicit residual |g_{i+1}| decides convergence.
 // Convergence as the reference (:600-611, :968-988): ||r|| / ||r0|| < tol or
 // ||r|| < 1000 eps (absolute); a system stops when its iterations reach
@@ -184,7 +185,101 @@ GmresOut gmres_solve(const Operator<TI>& op, const std::vector<T>& shifts, const
     };
     std::vector<TI> hact(kk, TI(1));
     if (cgs) upload(d_act, hact.data(), kk);
+    void *sig_h = nullptr, *sig_d = nullptr;
+    const bool devmode = cgs && rt().gmres_device && kk <= 64 && signal_buffer(&sig_h, &sig_d);
     int i = 0;
+    if (devmode) {
+      // ---- the cycle with the Givens rotations on the device: per step the product, the
+      // projection, the three sweeps, the rotations and the new basis vector are enqueued
+      // without waiting; the host then reads the active flags of the step before (the device
+      // works on this step meanwhile) and stops once no system is left ----
+      const size_t m1 = (size_t)m + 1, nhd = (size_t)kk * m1 + 3 * (size_t)kk;
+      const size_t nd = nhd + (size_t)kk * m * m1 + 2 * (size_t)kk * m;
+      const size_t sbytes = nd * sizeof(double) + 3 * (size_t)kk * sizeof(int);
+      const size_t inv_off = (sbytes + 15) / 16 * 16, abytes = inv_off + sizeof(TI) * (size_t)kk;
+      char* sbuf = static_cast<char*>(dev_alloc(abytes));
+      struct ReleaseState {
+        void* p;
+        size_t n;
+        ~ReleaseState() { dev_release(p, n); }
+      } release_state{sbuf, abytes};
+      GmresDevState ds;
+      double* dp = reinterpret_cast<double*>(sbuf);
+      ds.g = dp;
+      ds.normR0 = dp + (size_t)kk * m1;
+      ds.tol = ds.normR0 + kk;
+      ds.relres = ds.tol + kk;
+      ds.H = ds.relres + kk;
+      ds.cs = ds.H + (size_t)kk * m * m1;
+      ds.sn = ds.cs + (size_t)kk * m;
+      int* ip = reinterpret_cast<int*>(dp + nd);
+      ds.steps = ip;
+      ds.status = ip + kk;
+      ds.iters0 = ip + 2 * kk;
+      TI* d_inv = reinterpret_cast<TI*>(sbuf + inv_off);
+      std::vector<double> hd(nhd, 0.0);
+      std::vector<int> hi(3 * (size_t)kk, 0);
+      for (int a = 0; a < kk; a++) {
+        hd[(size_t)a * m1] = rn[a];
+        hd[(size_t)kk * m1 + a] = normR0[act[a]];
+        hd[(size_t)kk * m1 + kk + a] = tol[act[a]];
+        hd[(size_t)kk * m1 + 2 * kk + a] = out.relres[act[a]];
+        hi[kk + a] = 1;
+        hi[2 * kk + a] = out.iters[act[a]];
+      }
+      auto upload_bytes = [&](void* dst, const void* src, size_t bytes) {
+        if (!stage_upload_2d(dst, bytes, src, bytes, bytes, 1)) {
+          PHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, rt().stream));
+          PHX_HIP(hipStreamSynchronize(rt().stream));
+        }
+      };
+      upload_bytes(ds.g, hd.data(), sizeof(double) * nhd);
+      upload_bytes(ds.steps, hi.data(), sizeof(int) * hi.size());
+      uint32_t seqs[2] = {0, 0};
+      for (; i < m; i++) {
+        op.apply_shifted(TI(1), s.data(), vcur.get(), TI(0), w.get(), nullptr);
+        project(w.get());
+        const int nl = i + 1, nc = nl * kk;
+        TI* D1 = d_outs;
+        TI* D2 = d_outs + nc;
+        TI* N = d_outs + 2 * nc;
+        const TI* v = Vb->dev();
+        const int64_t ldv = Vb->lda;
+        gmres_cgs_dev<TI>(0, nrows, v, ldv, w->dev(), w->lda, kk, nl, nullptr, nullptr, dbuf, work_elems, D1);
+        gmres_cgs_dev<TI>(1, nrows, v, ldv, w->dev(), w->lda, kk, nl, D1, d_act, dbuf, work_elems, D2);
+        gmres_cgs_dev<TI>(2, nrows, v, ldv, w->dev(), w->lda, kk, nl, D2, d_act, dbuf, work_elems, N);
+        gmres_givens_dev<TI>(ds, kk, m, i, d_outs, d_act, d_inv, 1000 * eps, maxIter);
+        View<TI> vn(Vb.get(), (i + 1) * kk, (i + 2) * kk - 1);
+        gmres_normalize_dev<TI>(nrows, kk, w->dev(), w->lda, d_inv, N, vcur->dev(), vcur->lda, vn->dev(), vn->lda);
+        seqs[i % 2] = publish_async(d_act, sizeof(TI) * kk, 1 + i % 2);
+        PHX_REQUIRE(seqs[i % 2] != 0, PHIST_FUNCTIONAL_ERROR, "GMRES: the active flags could not be published");
+        if (i >= 1) {
+          const TI* f = static_cast<const TI*>(published_data(seqs[(i - 1) % 2], 1 + (i - 1) % 2));
+          bool any = false;
+          for (int a = 0; a < kk; a++) any = any || f[a] != TI(0);
+          if (!any) break;  // step i was enqueued already: for stopped systems it changes nothing
+        }
+      }
+      // the cycle's recurrences back to the host (one synchronisation per cycle)
+      std::vector<char> hs(sbytes);
+      PHX_HIP(hipMemcpyAsync(hs.data(), sbuf, sbytes, hipMemcpyDeviceToHost, rt().stream));
+      PHX_HIP(hipStreamSynchronize(rt().stream));
+      const double* hg = reinterpret_cast<const double*>(hs.data());
+      const double* hrel = hg + (size_t)kk * m1 + 2 * kk;
+      const double* hH = hrel + kk;
+      const int* hsteps = reinterpret_cast<const int*>(hg + nd);
+      const int* hstat = hsteps + kk;
+      for (int a = 0; a < kk; a++) {
+        const int j = act[a];
+        steps[a] = hsteps[a];
+        out.iters[j] += hsteps[a];
+        if (hsteps[a] > 0) out.relres[j] = hrel[a];
+        if (hstat[a] != 1) status[j] = hstat[a];
+        active[a] = hstat[a] == 1;
+        std::copy(hg + (size_t)a * m1, hg + (size_t)(a + 1) * m1, &g[(size_t)a * m1]);
+        std::copy(hH + (size_t)a * m * m1, hH + (size_t)(a + 1) * m * m1, &H[(size_t)a * m * m1]);
+      }
+    } else
     for (; i < m; i++) {
       bool live = false;
       for (int a = 0; a < kk; a++) live = live || active[a];

@@ -676,6 +676,108 @@ void gmres_cgs_dev(int mode, int64_t nrows, const T* v, int64_t ldv, T* w, int64
   PHX_HIP(hipGetLastError());
 }
 
+// One Arnoldi step's Givens rotations of the blocked GMRES on the device, one
+// thread per system a (the host loop's arithmetic, in double): from the
+// sweeps' sums outs = D1 | D2 | N (D1, D2: (i+1)*kk coefficients of the two
+// passes, N: the squared norms of w) it forms Hessenberg column i, applies
+// the system's earlier rotations and a new one, updates g, and decides
+// convergence (|g_{i+1}| < 1000 eps or |g_{i+1}| / ||r0|| < tol, or a zero
+// norm; the iteration limit; a non-finite norm fails the system); act[a]
+// (the sweeps' mask) and inv[a] (1 / ||w||, 0 for stopped systems) are
+// written for the next kernels.
+template <typename T>
+__global__ void __launch_bounds__(64) k_gmres_givens(GmresDevState st, int kk, int m, int i, const T* __restrict__ outs,
+                                                     T* act, T* inv, double eps1000, int max_iter) {
+  const int a = threadIdx.x;
+  if (a >= kk) return;
+  const int nc = (i + 1) * kk;
+  const double hn = sqrt((double)outs[2 * nc + a]);
+  const bool live = act[a] != T(0);
+  inv[a] = (live && hn > 0.0 && isfinite(hn)) ? T(1.0 / hn) : T(0);
+  if (!live) return;
+  if (!isfinite(hn)) {
+    act[a] = T(0);
+    st.status[a] = 3;
+    return;
+  }
+  const int m1 = m + 1;
+  double* h = st.H + ((size_t)a * m + i) * m1;  // column i of system a, rows 0..m
+  double* cs = st.cs + (size_t)a * m;
+  double* sn = st.sn + (size_t)a * m;
+  double* g = st.g + (size_t)a * m1;
+  for (int l = 0; l <= i; l++) h[l] = (double)outs[l * kk + a] + (double)outs[nc + l * kk + a];
+  h[i + 1] = hn;
+  for (int l = 0; l < i; l++) {
+    const double c = cs[l], sv = sn[l];
+    const double p0 = h[l], p1 = h[l + 1];
+    h[l] = c * p0 + sv * p1;
+    h[l + 1] = -sv * p0 + c * p1;
+  }
+  const double p0 = h[i], p1 = h[i + 1];
+  const double r = hypot(p0, p1);
+  const double c = r > 0.0 ? p0 / r : 1.0, sv = r > 0.0 ? p1 / r : 0.0;
+  cs[i] = c;
+  sn[i] = sv;
+  h[i] = r;
+  h[i + 1] = 0.0;
+  g[i + 1] = -sv * g[i];
+  g[i] = c * g[i];
+  st.steps[a] = i + 1;
+  const double nr = fabs(g[i + 1]);
+  const double rel = nr / st.normR0[a];
+  st.relres[a] = rel;
+  if (nr < eps1000 || rel < st.tol[a] || hn == 0.0) {
+    act[a] = T(0);
+    st.status[a] = 0;
+  } else if (st.iters0[a] + i + 1 >= max_iter) {
+    act[a] = T(0);
+    st.status[a] = 3;
+  }
+}
+
+// v = w * inv (column a by inv[a]; 0 where ||w(:,a)||^2 = nrm2[a] is not
+// finite, so a broken column cannot reach other systems as 0 * NaN) written
+// to both vcur and the basis block vb; kp (a power of two >= kk) lanes per row
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gmres_normalize(int64_t nrows, int kk, int kshift, const T* __restrict__ w,
+                                                            int64_t ldw, const T* __restrict__ inv,
+                                                            const T* __restrict__ nrm2, T* __restrict__ vcur,
+                                                            int64_t ldc, T* __restrict__ vb, int64_t ldv) {
+  const int a = threadIdx.x & ((1 << kshift) - 1);
+  if (a >= kk) return;
+  const T f = isfinite((double)nrm2[a]) ? inv[a] : T(0);
+  const bool zero = !isfinite((double)nrm2[a]);
+  const int64_t rpb = kBlock >> kshift;
+  for (int64_t r = (int64_t)blockIdx.x * rpb + (threadIdx.x >> kshift); r < nrows; r += (int64_t)gridDim.x * rpb) {
+    const T v = zero ? T(0) : f * w[r * ldw + a];
+    vcur[r * ldc + a] = v;
+    vb[r * ldv + a] = v;
+  }
+}
+
+template <typename T>
+void gmres_givens_dev(const GmresDevState& st, int kk, int m, int i, const T* outs, T* act, T* inv, double eps1000,
+                      int max_iter) {
+  PHX_REQUIRE(kk >= 1 && kk <= 64, PHIST_INVALID_INPUT, "%d systems (1..64)", kk);
+  hipLaunchKernelGGL(k_gmres_givens<T>, dim3(1), dim3(64), 0, rt().stream, st, kk, m, i, outs, act, inv, eps1000,
+                     max_iter);
+  PHX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void gmres_normalize_dev(int64_t nrows, int kk, const T* w, int64_t ldw, const T* inv, const T* nrm2, T* vcur,
+                         int64_t ldc, T* vb, int64_t ldv) {
+  if (nrows == 0) return;
+  int ks = 0;
+  while ((1 << ks) < kk) ks++;
+  PHX_REQUIRE(ks <= 6, PHIST_INVALID_INPUT, "%d systems (1..64)", kk);
+  const int64_t rpb = kBlock >> ks;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + rpb - 1) / rpb, (int64_t)rt().num_cu * 8));
+  hipLaunchKernelGGL(k_gmres_normalize<T>, dim3(grid), dim3(kBlock), 0, rt().stream, nrows, kk, ks, w, ldw, inv, nrm2,
+                     vcur, ldc, vb, ldv);
+  PHX_HIP(hipGetLastError());
+}
+
 // Y = X converted to TD (same shape; any row strides)
 template <typename TS, typename TD>
 void convert_block(const TS* x, int64_t ldx, TD* y, int64_t ldy, int64_t nrows, int ncols) {
@@ -707,6 +809,13 @@ template void gmres_cgs_dev<double>(int, int64_t, const double*, int64_t, double
                                     const double*, const double*, double*, int64_t, double*);
 template void gmres_cgs_dev<float>(int, int64_t, const float*, int64_t, float*, int64_t, int, int, const float*,
                                    const float*, float*, int64_t, float*);
+template void gmres_givens_dev<double>(const GmresDevState&, int, int, int, const double*, double*, double*, double,
+                                       int);
+template void gmres_givens_dev<float>(const GmresDevState&, int, int, int, const float*, float*, float*, double, int);
+template void gmres_normalize_dev<double>(int64_t, int, const double*, int64_t, const double*, const double*, double*,
+                                          int64_t, double*, int64_t);
+template void gmres_normalize_dev<float>(int64_t, int, const float*, int64_t, const float*, const float*, float*,
+                                         int64_t, float*, int64_t);
 template void convert_block<double, float>(const double*, int64_t, float*, int64_t, int64_t, int);
 template void convert_block<float, double>(const float*, int64_t, double*, int64_t, int64_t, int);
 

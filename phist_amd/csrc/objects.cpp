@@ -350,7 +350,7 @@ void sd_from_device(const SdMat<T>* M) {
   const size_t span = (size_t)ld * (M->ncols - 1) + M->nrows;
   if (M->nrows == ld) {
     if (fetch_to_host(M->host(), M->dev(), sizeof(T) * span)) return;
-  } else if (sizeof(T) * span <= Runtime::kSignalBytes - Runtime::kSignalData) {
+  } else if (sizeof(T) * span <= Runtime::kSignalBytes - Runtime::kSignalData - 2 * kSignalSlotBytes) {
     std::vector<T> tmp(span);
     if (fetch_to_host(tmp.data(), M->dev(), sizeof(T) * span)) {
       for (int c = 0; c < M->ncols; c++)

@@ -100,6 +100,7 @@ void ensure_init() {
   if (const char* s = getenv("PHIST_AMD_MIX_MFMA")) r.mix_mfma = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_MINRES_DEVICE")) r.minres_device = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GMRES_CGS")) r.gmres_cgs = atoi(s) != 0;
+  if (const char* s = getenv("PHIST_AMD_GMRES_DEVICE")) r.gmres_device = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_SERVE_MAX_REPS")) r.serve_max_reps = std::max(1, atoi(s));
   if (const char* s = getenv("PHIST_AMD_GRAM_ROWS")) r.gram_rows = atoi(s) != 0;
   if (const char* s = getenv("PHIST_AMD_GRAM_SETS")) r.gram_sets = atoi(s) != 0;
@@ -185,15 +186,16 @@ void wait_signal(uint32_t seq) {
   Runtime& r = rt();
   const uint32_t* word = static_cast<const uint32_t*>(r.h_sig);
   const auto t0 = std::chrono::steady_clock::now();
+  // the numbers grow in stream order: a later one means this one was written too
+  auto reached = [&] { return (int32_t)(__atomic_load_n(word, __ATOMIC_ACQUIRE) - seq) >= 0; };
   for (int spin = 1;; spin++) {
-    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+    if (reached()) return;
     if ((spin & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) break;
     __builtin_ia32_pause();
   }
   // long waits (much queued work) block in the runtime instead of spinning
   PHX_HIP(hipStreamSynchronize(r.stream));
-  PHX_REQUIRE(__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq, PHIST_FUNCTIONAL_ERROR,
-              "published reduction %u not seen after the stream finished", seq);
+  PHX_REQUIRE(reached(), PHIST_FUNCTIONAL_ERROR, "published reduction %u not seen after the stream finished", seq);
 }
 
 namespace {
@@ -220,14 +222,16 @@ __global__ void __launch_bounds__(64) k_publish_words(const W* __restrict__ src,
 }
 }  // namespace
 
-bool fetch_to_host(void* host, const void* dev, size_t bytes) {
+namespace {
+// enqueues the publication of bytes from dev at byte offset off of the buffer; the sequence number, or 0
+uint32_t publish_at(const void* dev, size_t bytes, size_t off, void** host_area) {
   void *hs = nullptr, *ds = nullptr;
-  if (bytes == 0 || bytes > Runtime::kSignalBytes - Runtime::kSignalData) return false;
   const uintptr_t al = reinterpret_cast<uintptr_t>(dev) | bytes;
-  if (al % 4 != 0 || !signal_buffer(&hs, &ds)) return false;
+  if (bytes == 0 || al % 4 != 0 || !signal_buffer(&hs, &ds)) return 0;
   Runtime& r = rt();
-  const uint32_t seq = ++r.sig_seq;
-  char* pub = static_cast<char*>(ds) + Runtime::kSignalData;
+  uint32_t seq = ++r.sig_seq;
+  if (seq == 0) seq = ++r.sig_seq;  // 0 means "not published"
+  char* pub = static_cast<char*>(ds) + off;
   uint32_t* word = static_cast<uint32_t*>(ds);
   typedef unsigned int w4_t __attribute__((ext_vector_type(4)));
   if (al % 16 == 0)
@@ -237,9 +241,33 @@ bool fetch_to_host(void* host, const void* dev, size_t bytes) {
     hipLaunchKernelGGL(k_publish_words<uint32_t>, dim3(1), dim3(64), 0, r.stream, static_cast<const uint32_t*>(dev),
                        (int64_t)(bytes / 4), reinterpret_cast<uint32_t*>(pub), word, seq);
   PHX_HIP(hipGetLastError());
+  *host_area = static_cast<char*>(hs) + off;
+  return seq;
+}
+}  // namespace
+
+// the buffer: sequence word, then slot 0 (fetch_to_host, up to the end minus two slots), then slots 1, 2
+bool fetch_to_host(void* host, const void* dev, size_t bytes) {
+  if (bytes > Runtime::kSignalBytes - Runtime::kSignalData - 2 * kSignalSlotBytes) return false;
+  void* area = nullptr;
+  const uint32_t seq = publish_at(dev, bytes, Runtime::kSignalData, &area);
+  if (!seq) return false;
   wait_signal(seq);
-  memcpy(host, static_cast<const char*>(hs) + Runtime::kSignalData, bytes);
+  memcpy(host, area, bytes);
   return true;
+}
+
+uint32_t publish_async(const void* dev, size_t bytes, int slot) {
+  PHX_REQUIRE(slot == 1 || slot == 2, PHIST_INVALID_INPUT, "published slot %d (1 or 2)", slot);
+  if (bytes > kSignalSlotBytes) return 0;
+  void* area = nullptr;
+  return publish_at(dev, bytes, Runtime::kSignalBytes - (size_t)(3 - slot) * kSignalSlotBytes, &area);
+}
+
+const void* published_data(uint32_t seq, int slot) {
+  PHX_REQUIRE(seq != 0 && (slot == 1 || slot == 2), PHIST_INVALID_INPUT, "published slot %d, number %u", slot, seq);
+  wait_signal(seq);
+  return static_cast<const char*>(rt().h_sig) + Runtime::kSignalBytes - (size_t)(3 - slot) * kSignalSlotBytes;
 }
 
 void sync_stream() { PHX_HIP(hipStreamSynchronize(rt().stream)); }
@@ -861,6 +889,13 @@ void phist_amd_set_spmv_stage(int mode, int* iflag) {
     ensure_init();
     PHX_REQUIRE(mode >= -1 && mode <= 1, PHIST_INVALID_INPUT, "staging mode %d: -1, 0 or 1", mode);
     rt().spmv_stage = mode;
+    *iflag = 0;
+  });
+}
+void phist_amd_set_gmres_device(int enable, int* iflag) {
+  guard(iflag, __func__, [&] {
+    ensure_init();
+    rt().gmres_device = enable != 0;
     *iflag = 0;
   });
 }

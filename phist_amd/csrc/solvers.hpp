@@ -34,6 +34,14 @@ template <typename T>
 void gmres_cgs_dev(int mode, int64_t nrows, const T* v, int64_t ldv, T* w, int64_t ldw, int kk, int nl,
                    const T* coef, const T* act, T* work, int64_t work_elems, T* out);
 
+// device Givens step and basis normalisation of the blocked GMRES (krylov.hip; GmresDevState in amd_common.hpp)
+template <typename T>
+void gmres_givens_dev(const GmresDevState& st, int kk, int m, int i, const T* outs, T* act, T* inv, double eps1000,
+                      int max_iter);
+template <typename T>
+void gmres_normalize_dev(int64_t nrows, int kk, const T* w, int64_t ldw, const T* inv, const T* nrm2, T* vcur,
+                         int64_t ldc, T* vb, int64_t ldv);
+
 // owned block vector, freed at scope exit
 template <typename T>
 struct MvBox {

@@ -117,3 +117,42 @@ def test_gmres_rejects_bad_input(be64):
         be64.gmres_shifted(A, [0.0, 1.0], B, X, 1e-8, 10)
     for o in (B, X, A):
         o.delete()
+
+
+@pytest.mark.parametrize("restart,nsys,tol", [(30, 4, 1e-10), (8, 4, 1e-10), (12, 40, 1e-9), (20, 1, 1e-10),
+                                              (30, 4, 1e-3)])
+def test_gmres_device_givens_matches_host(be64, restart, nsys, tol):
+    """the Givens rotations on the device (the host one Arnoldi step ahead) and on the host give the same
+    iteration counts and solutions to rounding, in FP64 and in FP32 on the cached copy, with systems
+    converging at different steps and across restarts; both reach the tolerance by the true residual"""
+    import ctypes
+    fl = ctypes.c_int(0)
+    A = be64.banded_random(20000, 16, 300, 20.0, 5)
+    n = A.nrows
+    shifts = np.array([0.0, 1.5, -3.0, 6.0]) if nsys == 4 else np.linspace(-3.0, 6.0, nsys)
+    Bh = np.random.default_rng(7).uniform(-1, 1, (n, nsys))
+    for fp32 in (False, True):
+        t = max(tol, 1e-5) if fp32 else tol
+        res = {}
+        try:
+            for dev in (0, 1):
+                be64.L.phist_amd_set_gmres_device(dev, ctypes.byref(fl))
+                assert fl.value == 0
+                B, X = be64.mvec(A.domain_map, nsys, Bh), be64.mvec(A.domain_map, nsys)
+                it, rr, unconv = be64.gmres_shifted(A, shifts, B, X, t, 500, restart=restart, fp32=fp32)
+                res[dev] = (np.array(it), unconv, X.get())
+                B.delete()
+                X.delete()
+        finally:
+            be64.L.phist_amd_set_gmres_device(1, ctypes.byref(fl))
+        (it0, u0, x0), (it1, u1, x1) = res[0], res[1]
+        assert u0 == 0 and u1 == 0
+        assert np.all(np.abs(it1 - it0) <= 1), (it0, it1)
+        got = true_relres(A, shifts, Bh, x1)
+        assert np.all(got < 2 * t), got
+        # the same steps give the same solution to rounding; a convergence decided one step apart
+        # (a last-bit difference of the rotations at the threshold) leaves both within the tolerance
+        scale = np.linalg.norm(x0, axis=0)
+        bound = np.where(it1 == it0, 1e-4 if fp32 else 1e-8, 20 * t)
+        assert np.all(np.linalg.norm(x1 - x0, axis=0) <= bound * scale), (it0, it1)
+    A.delete()
