@@ -737,21 +737,46 @@ __global__ void __launch_bounds__(64) k_gmres_givens(GmresDevState st, int kk, i
 
 // v = w * inv (column a by inv[a]; 0 where ||w(:,a)||^2 = nrm2[a] is not
 // finite, so a broken column cannot reach other systems as 0 * NaN) written
-// to both vcur and the basis block vb; kp (a power of two >= kk) lanes per row
-template <typename T>
+// to both vcur and the basis block vb.  A lane owns VS consecutive systems of
+// a row (16-byte vectors where kk and the row strides allow, else one), kp
+// lanes per row (a power of two), and four rows per tile whose loads all go
+// out before the stores.
+template <typename T, int VS>
 __global__ void __launch_bounds__(kBlock) k_gmres_normalize(int64_t nrows, int kk, int kshift, const T* __restrict__ w,
                                                             int64_t ldw, const T* __restrict__ inv,
                                                             const T* __restrict__ nrm2, T* __restrict__ vcur,
                                                             int64_t ldc, T* __restrict__ vb, int64_t ldv) {
-  const int a = threadIdx.x & ((1 << kshift) - 1);
-  if (a >= kk) return;
-  const T f = isfinite((double)nrm2[a]) ? inv[a] : T(0);
-  const bool zero = !isfinite((double)nrm2[a]);
+  constexpr int U = 4;
+  const int q = threadIdx.x & ((1 << kshift) - 1), a0 = q * VS;
+  if (a0 >= kk) return;
+  T f[VS];
+#pragma unroll
+  for (int e = 0; e < VS; e++) {
+    const bool ok = isfinite((double)nrm2[a0 + e]);
+    f[e] = ok ? inv[a0 + e] : T(0);
+  }
+  bool zero[VS];
+#pragma unroll
+  for (int e = 0; e < VS; e++) zero[e] = !isfinite((double)nrm2[a0 + e]);
   const int64_t rpb = kBlock >> kshift;
-  for (int64_t r = (int64_t)blockIdx.x * rpb + (threadIdx.x >> kshift); r < nrows; r += (int64_t)gridDim.x * rpb) {
-    const T v = zero ? T(0) : f * w[r * ldw + a];
-    vcur[r * ldc + a] = v;
-    vb[r * ldv + a] = v;
+  const int64_t rl = threadIdx.x >> kshift;
+  for (int64_t r0 = (int64_t)blockIdx.x * rpb * U; r0 < nrows; r0 += (int64_t)gridDim.x * rpb * U) {
+    T x[U][VS];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t r = r0 + u * rpb + rl;
+      if (r < nrows) vload<T, VS>(w + r * ldw + a0, x[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t r = r0 + u * rpb + rl;
+      if (r >= nrows) continue;
+      T v[VS];
+#pragma unroll
+      for (int e = 0; e < VS; e++) v[e] = zero[e] ? T(0) : f[e] * x[u][e];
+      vstore<T, VS>(vcur + r * ldc + a0, v);
+      vstore<T, VS>(vb + r * ldv + a0, v);
+    }
   }
 }
 
@@ -768,13 +793,22 @@ template <typename T>
 void gmres_normalize_dev(int64_t nrows, int kk, const T* w, int64_t ldw, const T* inv, const T* nrm2, T* vcur,
                          int64_t ldc, T* vb, int64_t ldv) {
   if (nrows == 0) return;
+  PHX_REQUIRE(kk >= 1 && kk <= 64, PHIST_INVALID_INPUT, "%d systems (1..64)", kk);
+  constexpr int V = 16 / sizeof(T);
+  auto al = [](const void* ptr, int64_t ld) { return ((uintptr_t)ptr % 16) == 0 && (ld * (int64_t)sizeof(T)) % 16 == 0; };
+  const bool vec = kk % V == 0 && al(w, ldw) && al(vcur, ldc) && al(vb, ldv);
+  const int lanes = vec ? kk / V : kk;
   int ks = 0;
-  while ((1 << ks) < kk) ks++;
-  PHX_REQUIRE(ks <= 6, PHIST_INVALID_INPUT, "%d systems (1..64)", kk);
-  const int64_t rpb = kBlock >> ks;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + rpb - 1) / rpb, (int64_t)rt().num_cu * 8));
-  hipLaunchKernelGGL(k_gmres_normalize<T>, dim3(grid), dim3(kBlock), 0, rt().stream, nrows, kk, ks, w, ldw, inv, nrm2,
-                     vcur, ldc, vb, ldv);
+  while ((1 << ks) < lanes) ks++;
+  const int64_t rows_per_tile = (int64_t)(kBlock >> ks) * 4;
+  const int grid =
+      (int)std::max<int64_t>(1, std::min<int64_t>((nrows + rows_per_tile - 1) / rows_per_tile, (int64_t)rt().num_cu * 8));
+  if (vec)
+    hipLaunchKernelGGL((k_gmres_normalize<T, V>), dim3(grid), dim3(kBlock), 0, rt().stream, nrows, kk, ks, w, ldw, inv,
+                       nrm2, vcur, ldc, vb, ldv);
+  else
+    hipLaunchKernelGGL((k_gmres_normalize<T, 1>), dim3(grid), dim3(kBlock), 0, rt().stream, nrows, kk, ks, w, ldw, inv,
+                       nrm2, vcur, ldc, vb, ldv);
   PHX_HIP(hipGetLastError());
 }
 

@@ -123,8 +123,9 @@ def test_gmres_rejects_bad_input(be64):
                                               (30, 4, 1e-3)])
 def test_gmres_device_givens_matches_host(be64, restart, nsys, tol):
     """the Givens rotations on the device (the host one Arnoldi step ahead) and on the host give the same
-    iteration counts and solutions to rounding, in FP64 and in FP32 on the cached copy, with systems
-    converging at different steps and across restarts; both reach the tolerance by the true residual"""
+    iteration counts (within one) and solutions to rounding, in FP64 and in FP32 on the cached copy, with
+    systems converging at different steps and across restarts; both reach the tolerance by the true
+    residual"""
     import ctypes
     fl = ctypes.c_int(0)
     A = be64.banded_random(20000, 16, 300, 20.0, 5)
