@@ -1,7 +1,7 @@
 """CPU Gram of a tall-skinny block for the CPU references of the Gram and
 CholQR benches.
 
-TEST INFRASTRUCTURE / CPU BASELINE ONLY (bench.py's cpu_baseline leg and
+TEST INFRASTRUCTURE AND CPU REFERENCE TIMING ONLY (bench.py's cpu_baseline leg and
 tests/); the product library never imports, links or runs it.
 
 The reference's builtin Gram (gemm_kernels_sC.F90:17-265: an OpenMP loop

@@ -6,7 +6,7 @@
  * stores, 3 * 8 bytes per element).  That file needs PHIST's CMake-generated
  * phist_config.h and does not build here; this is a restatement.
  *
- * TEST INFRASTRUCTURE / CPU BASELINE ONLY: bench.py's cpu_baseline leg calls
+ * TEST INFRASTRUCTURE AND CPU REFERENCE TIMING ONLY: bench.py's cpu_baseline leg calls
  * it; the product library never links or runs it.
  */
 #include <emmintrin.h>

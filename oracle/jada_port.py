@@ -3,7 +3,7 @@ correction solves, for the CPU reference time of the JD time-to-solution
 configuration (3D Laplacian, 8 smallest pairs, blockSize 4, minBas 16,
 maxBas 32, at most 10 GMRES iterations per correction, tol 1e-8).
 
-TEST INFRASTRUCTURE / CPU BASELINE ONLY.  bench.py's cpu_baseline leg times
+TEST INFRASTRUCTURE AND CPU REFERENCE TIMING ONLY.  bench.py's cpu_baseline leg times
 it and tests/ check it; the product library never imports, links or runs it.
 
 The reference's own JD (src/jada/phist_subspacejada_def.hpp) needs MPI and

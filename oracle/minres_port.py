@@ -2,7 +2,7 @@
 batched shifted MINRES configuration (honeycomb sheet, 32 shifts, 32 unit
 right-hand sides, tol 1e-6).
 
-TEST INFRASTRUCTURE / CPU BASELINE ONLY.  bench.py's cpu_baseline leg times
+TEST INFRASTRUCTURE AND CPU REFERENCE TIMING ONLY.  bench.py's cpu_baseline leg times
 it and tests/ check it; the product library never imports, links or runs it.
 
 k systems (A + s_j I) x_j = b_j iterate together, each with its own Lanczos
