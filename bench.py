@@ -229,6 +229,112 @@ def tall_skinny(be, args):
     print(json.dumps(out), flush=True)
 
 
+def blas1_bench(be, args):
+    """column dots <V, W>, norms ||V|| and Y = a X + b Y (mvec_dot_mvec, mvec_norm2, mvec_vadd_mvec with
+    per-column a) on N = 16,777,216 rows x k FP64 columns; byte models 2 N k 8, N k 8, 3 N k 8; the
+    kernel time is the library's HIP-event timer over the row kernel (the dots' fixed-order second
+    stage is not in it)"""
+    L = be.L
+    iflag = ctypes.c_int(0)
+    N, k = 16777216, args.k
+    m = be.map(N)
+    v, w = be.mvec(m, k), be.mvec(m, k)
+    v.random()
+    w.random()
+    out_h = np.zeros(k)
+    alpha = np.linspace(0.5, 1.5, k)
+    op = args.op
+    if op == "dot":
+        nbytes = 2 * N * k * 8
+
+        def step():
+            be.call("mvec_dot_mvec", v.h, w.h, out_h.ctypes.data_as(ctypes.c_void_p))
+    elif op == "norm2":
+        nbytes = N * k * 8
+
+        def step():
+            be.call("mvec_norm2", v.h, out_h.ctypes.data_as(ctypes.c_void_p))
+    else:
+        nbytes = 3 * N * k * 8
+
+        def step():
+            be.call("mvec_vadd_mvec", alpha.ctypes.data_as(ctypes.c_void_p), v.h, be.s(0.5), w.h)
+    for _ in range(args.warmup):
+        step()
+    be.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    be.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    L.phist_amd_kernel_timer(1, ctypes.byref(iflag))
+    for _ in range(args.steps):
+        step()
+    be.synchronize()
+    kms, kcount = ctypes.c_double(0.0), ctypes.c_int64(0)
+    L.phist_amd_kernel_timer_read(ctypes.byref(kms), ctypes.byref(kcount), ctypes.byref(iflag))
+    L.phist_amd_kernel_timer(0, ctypes.byref(iflag))
+    k_ms = kms.value / max(1, kcount.value)
+    gbs = nbytes / (k_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = blas1_cpu_baseline(op, v.get(), w.get(), alpha, nbytes)
+    out = {"metric": f"{op} GB/s (N=16777216, k={k}, FP64)", "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic (KISS random block vectors)",
+           "config": {"workload": f"{op} on 16,777,216 x {k} U(-1,1)", "bytes_per_op": nbytes,
+                      "kernel_ms_per_op": round(k_ms, 4), "kernel_launches_timed": kcount.value},
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)},
+           "cpu_baseline": cpu}
+    for o in (v, w):
+        o.delete()
+    m.delete()
+    print(json.dumps(out), flush=True)
+
+
+def blas1_cpu_baseline(op, Vh, Wh, alpha, nbytes, reps=5):
+    """the same operation on the host by the reference's flang-built OpenMP leaf kernels (ddot_k,
+    dnrm2_k, daxpby_k for k = 1, 2, 4, 8; row-major blocks are their column-major k x N arrays), median
+    of `reps` after one untimed run; None for other widths or without that build"""
+    import oracle_lib as ol
+    ref = ol.ref_leaf()
+    k = Vh.shape[1]
+    if ref is None or k not in (1, 2, 4, 8):
+        return None
+    I, D = ctypes.c_int, ctypes.c_double
+    p = ol.p
+    n = Vh.shape[0]
+    V = np.ascontiguousarray(Vh)
+    W = np.ascontiguousarray(Wh)
+    res = np.zeros(k)
+    al = np.ascontiguousarray(alpha, np.float64)
+    if op == "dot":
+        f = getattr(ref, f"ddot_{k}_")
+        run = lambda: f(ctypes.byref(I(n)), p(V), p(W), p(res))
+    elif op == "norm2":
+        f = getattr(ref, f"dnrm2_{k}_")
+        run = lambda: f(ctypes.byref(I(n)), p(V), p(res))
+    else:
+        f = getattr(ref, f"daxpby_{k}_") if k > 1 else ref.daxpby_1_
+        if k == 1:
+            run = lambda: f(ctypes.byref(I(n)), ctypes.byref(D(float(al[0]))), p(V), ctypes.byref(D(0.5)), p(W))
+        else:
+            run = lambda: f(ctypes.byref(I(n)), p(al), p(V), ctypes.byref(D(0.5)), p(W))
+    run()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(nbytes / t / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "reference",
+            "sample": f"the whole 16,777,216 x {k} operation by the reference's {f.__name__.rstrip('_')} "
+                      f"(flang, OpenMP), median of {reps}",
+            "ms_per_op": round(t * 1e3, 3), **cpu_info_fields(threads)}
+
+
 def tall_skinny_cpu_baseline(op, Vh, nbytes, reps=5):
     """the same operation on the host cores, same byte model, median of `reps` after one untimed run:
     gram: V'V by OpenBLAS (the reference's gemm_kernels_sC.F90 needs its CMake-generated header);
@@ -956,12 +1062,12 @@ def main():
     ap.add_argument("--rows", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--op", choices=["spmv", "gram", "cholqr", "mvsd", "jada", "minres", "serve", "carp", "gmres",
-                                     "tune"],
+                                     "tune", "dot", "norm2", "axpby"],
                     default="spmv",
                     help="spmv (default): the fused SpMV line; gram / cholqr / mvsd (fused_mvsd_mvTmv, "
                          "W = V*C and W'W, beta=0): the tall-skinny configuration (16,777,216 x k, U(-1,1)) "
                          "reported in the same format")
-    ap.add_argument("--k", type=int, default=8, help="block width for --op gram/cholqr")
+    ap.add_argument("--k", type=int, default=8, help="block width for --op gram/cholqr/mvsd/dot/norm2/axpby")
     ap.add_argument("--rate", type=float, default=40.0, help="--op serve: mean request arrivals per second")
     ap.add_argument("--duration", type=float, default=20.0, help="--op serve: seconds of arrivals")
     ap.add_argument("--starts", type=int, default=5, help="--op jada: start vectors averaged per precision")
@@ -1033,6 +1139,8 @@ def main():
         return gmres_bench(be, args)
     if args.op == "tune":
         return tune_bench(be, args)
+    if args.op in ("dot", "norm2", "axpby"):
+        return blas1_bench(be, args)
     if args.op != "spmv":
         return tall_skinny(be, args)
     L.phist_amd_sparseMat_set_format(32, 256, ctypes.byref(iflag))

@@ -159,6 +159,7 @@ int launch_kind(Kind kind, RowArgs<T>& p) {
   const int64_t rows_per_tile = kind == K_DOT ? RPB : (int64_t)RPB * kRowUnroll;
   p.ntiles = (p.nrows + rows_per_tile - 1) / rows_per_tile;
   int grid;
+  ktimer_begin();
   switch (kind) {
     case K_AXPBY:
       grid = persistent_blocks(k_axpby<T, VPL, LPR>, 0, p.ntiles);
@@ -175,6 +176,7 @@ int launch_kind(Kind kind, RowArgs<T>& p) {
       break;
   }
   PHX_HIP(hipGetLastError());
+  ktimer_end();
   return grid;
 }
 

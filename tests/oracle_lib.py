@@ -50,12 +50,36 @@ def ref_leaf():
     return ctypes.CDLL(REF_SO)
 
 
+def _process_cpus():
+    """the CPUs of this process's cgroup (a thread's own affinity can be narrower)"""
+    try:
+        with open("/sys/fs/cgroup/cpuset.cpus.effective") as f:
+            cpus = set()
+            for part in f.read().strip().split(","):
+                if part:
+                    a, _, b = part.partition("-")
+                    cpus.update(range(int(a), int(b or a) + 1))
+            if cpus:
+                return cpus
+    except OSError:
+        pass
+    return set(range(os.cpu_count() or 1))
+
+
 def host_stream_triad(n=1 << 27, reps=10):
     """the host's STREAM triad z = x + a*y over n doubles on the OpenMP threads
     (oracle/host_stream.c): (best, mean) GB/s of reps runs at 24 bytes per
-    element, or None if that library is not built"""
+    element, or None if that library is not built.  An OpenMP runtime that ran
+    before in this process (the reference's flang-built kernels, with
+    OMP_PROC_BIND) may have bound the calling thread to one core, and the
+    triad's runtime takes its thread places from that thread's affinity: the
+    affinity is widened to the process's CPUs first."""
     if not os.path.exists(STREAM_SO):
         return None
+    try:
+        os.sched_setaffinity(0, _process_cpus())
+    except OSError:
+        pass
     lib = ctypes.CDLL(STREAM_SO)
     best, mean = ctypes.c_double(), ctypes.c_double()
     rc = lib.host_stream_triad(ctypes.c_int64(n), ctypes.c_int(reps), ctypes.byref(best), ctypes.byref(mean))
