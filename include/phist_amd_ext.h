@@ -120,10 +120,11 @@ void phist_amd_synchronize(int* iflag);
 void phist_amd_timer_start(int* iflag);
 void phist_amd_timer_stop(float* ms, int* iflag);
 void phist_amd_device_info(char* name, int name_len, int* num_cu, int* iflag);
-/* per-kernel timer: with enable != 0 the main kernel of every SpMV and
- * tall-skinny GEMM call is bracketed by HIP events on the library stream
- * (synchronising after it); read returns the summed milliseconds and the
- * number of timed launches since the last enable */
+/* per-kernel timer: with enable = 1 the main kernel of every SpMV,
+ * tall-skinny GEMM and BLAS-1 call is bracketed by HIP events on the library
+ * stream (synchronising after it); read returns the summed milliseconds and
+ * the number of timed launches since the last enable.  0 stops it; 2 pauses
+ * and 3 resumes it without clearing the totals (to leave a call out). */
 void phist_amd_kernel_timer(int enable, int* iflag);
 void phist_amd_kernel_timer_read(double* total_ms, int64_t* count, int* iflag);
 /* known-size gather: every seg_bytes-byte segment (16..256, a power of two)

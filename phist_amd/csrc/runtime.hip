@@ -966,9 +966,14 @@ void phist_amd_timer_stop(float* ms, int* iflag) {
 void phist_amd_kernel_timer(int enable, int* iflag) {
   guard(iflag, __func__, [&] {
     ensure_init();
-    rt().ktimer_on = enable != 0;
-    rt().ktimer_ms = 0.0;
-    rt().ktimer_count = 0;
+    PHX_REQUIRE(enable >= 0 && enable <= 3, PHIST_INVALID_INPUT, "kernel timer mode %d (0..3)", enable);
+    if (enable >= 2) {  // pause / resume: the totals stay
+      rt().ktimer_on = enable == 3;
+    } else {
+      rt().ktimer_on = enable != 0;
+      rt().ktimer_ms = 0.0;
+      rt().ktimer_count = 0;
+    }
     *iflag = 0;
   });
 }

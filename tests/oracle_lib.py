@@ -12,6 +12,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libbuiltin_leaf.so")
 STREAM_SO = os.path.join(ORACLE_DIR, "libhost_stream.so")
+GRAM_SO = os.path.join(ORACLE_DIR, "libhost_gram.so")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 MATRICES = os.path.join(GOLDEN, "matrices")
 
@@ -86,6 +87,28 @@ def host_stream_triad(n=1 << 27, reps=10):
     if rc != 0:
         raise RuntimeError(f"host_stream_triad returned {rc}")
     return best.value, mean.value
+
+
+def host_gram(V, W=None):
+    """V'W (k x m) of row-major FP64 blocks V (n x k) and W (n x m; None: V) on the host's OpenMP threads
+    (oracle/host_gram.c, the row loop of the reference's gemm_kernels_sC.F90), or None if that library
+    is not built; the calling thread's affinity is widened first as for host_stream_triad"""
+    if not os.path.exists(GRAM_SO):
+        return None
+    try:
+        os.sched_setaffinity(0, _process_cpus())
+    except OSError:
+        pass
+    lib = ctypes.CDLL(GRAM_SO)
+    V = np.ascontiguousarray(V, np.float64)
+    W = V if W is None else np.ascontiguousarray(W, np.float64)
+    n, k = V.shape
+    m = W.shape[1]
+    M = np.zeros((m, k))  # column-major k x m = row-major m x k
+    rc = lib.host_gram(p(V), p(W), ctypes.c_int64(n), ctypes.c_int(k), ctypes.c_int(m), p(M))
+    if rc != 0:
+        raise RuntimeError(f"host_gram returned {rc}")
+    return M.T.copy()
 
 
 class Crs:
