@@ -393,7 +393,8 @@ struct SparseMat {
 
 // the staging tables of A for tiles of rpb SELL rows (built on first use:
 // host work over every slot, then one upload); nullptr when some tile
-// gathers from more intervals or window rows than SellStage allows
+// gathers from more intervals or window rows than SellStage allows, or when
+// they are not built yet inside an allocation-free call or a graph capture
 template <typename T> const SellStage* sell_stage(const SparseMat<T>* A, int rpb);
 // LDS a staged SpMV window may take per workgroup (two workgroups per CU still fit)
 constexpr int kStageLdsBytes = 64 << 10;

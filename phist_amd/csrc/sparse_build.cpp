@@ -206,6 +206,8 @@ const SellStage* sell_stage(const SparseMat<T>* A, int rpb) {
   std::lock_guard<std::mutex> lock(mu);
   auto found = A->stage.find(rpb);
   if (found != A->stage.end()) return found->second.ok ? &found->second : nullptr;
+  // building the tables allocates device memory: not inside an allocation-free call or a graph capture
+  if (rt().fixed_work || rt().capturing) return nullptr;
   SellStage& S = A->stage[rpb];  // not ok until the tables are up
   const int64_t nslots = A->nchunks * A->C;
   const int64_t ntiles = rpb > 0 ? (nslots + rpb - 1) / rpb : 0;

@@ -114,3 +114,44 @@ def test_noalloc_rejects_oversized_blocks(be64):
     for o in (d, x, w, A):
         o.delete()
     m.delete()
+
+
+@pytest.mark.parametrize("nvec", [4, 8])
+def test_noalloc_with_staging_forced(be64, nvec):
+    """with x staging forced on (phist_amd_set_spmv_stage(1)) the entry point still allocates nothing:
+    before the matrix's staging tables exist it runs unstaged (same W bit for bit, dots to rounding);
+    once a regular call has built them it stages too and matches that call bit for bit"""
+    fl = ctypes.c_int(0)
+    A = be64.laplace3d(20, 21, 22)
+    n = A.nrows
+    rng = np.random.default_rng(nvec)
+    X = rng.uniform(-1, 1, (n, nvec))
+    m = be64.map(n)
+    x = be64.mvec(m, nvec, X)
+    staged = lambda: (be64.L.phist_amd_spmv_staged_launches(ctypes.byref(c), ctypes.byref(fl)), c.value)[1]
+    c = ctypes.c_int64(0)
+    try:
+        be64.L.phist_amd_set_spmv_stage(1, ctypes.byref(fl))
+        assert fl.value == 0
+        wn1, wr, wn2 = be64.mvec(m, nvec), be64.mvec(m, nvec), be64.mvec(m, nvec)
+        d1, v1 = DevArray(be64, nvec), DevArray(be64, nvec)
+        s0 = staged()
+        assert noalloc(be64, 1.0, A, x, 0.0, wn1, d1, v1) == 0
+        assert staged() == s0  # no tables yet: not staged
+        wwr, vwr = be64.fused_spmv_mvdot(1.0, A, x, 0.0, wr)
+        assert staged() == s0 + 1  # the regular call built the tables and staged
+        d2, v2 = DevArray(be64, nvec), DevArray(be64, nvec)
+        assert noalloc(be64, 1.0, A, x, 0.0, wn2, d2, v2) == 0
+        assert staged() == s0 + 2
+        Wr = wr.get()
+        assert np.array_equal(wn1.get(), Wr) and np.array_equal(wn2.get(), Wr)
+        assert np.array_equal(d2.get(), wwr) and np.array_equal(v2.get(), vwr)
+        assert np.max(np.abs(d1.get() - wwr) / np.abs(wwr)) <= 1e-13
+        assert np.max(np.abs(v1.get() - vwr) / np.abs(vwr)) <= 1e-13
+        for o in (wn1, wr, wn2, d1, v1, d2, v2):
+            o.delete()
+    finally:
+        be64.L.phist_amd_set_spmv_stage(-1, ctypes.byref(fl))
+    x.delete()
+    A.delete()
+    m.delete()
